@@ -1,0 +1,116 @@
+"""Named scan configurations, written as prm (YAML) files.
+
+Units follow the prm interface: cm, 1/cm, radians, degrees for angles of
+files.  Every configuration uses FileType-0 angle/table conventions and
+synthetic raw data (optional `synthetic*` keys), so the files also parse with
+the reference's key set.
+
+  micro            8x8x3 volume, 4 x 16 detector: small enough for dense checks
+  tiny, tiny_ffs   test geometries the CPU oracle runs in well under a second
+  config0          16 x 368 detector, 576 views/turn, 128x128x32 volume,
+                   4 rotations (2 fail the reference's own z-range check),
+                   quadratic lambda 1e3, weights = counts, I0 = 1e5
+  config1          64 x 736 detector, in-plane FFS (2304 views/turn),
+                   512x512x96 volume (0.7 mm pixels), 3 rotations,
+                   edge-preserving lambda 5e3, delta 0.02/cm, I0 = 2e5
+  config2          config1 with the FP16 SSM and an FP64 resync every 2 its
+  config3          config1 geometry cut to 256x256x32 (1.4 mm pixels)
+  config4          512x512x256, 6 rotations, lung texture, I0 = 1e5
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+_BASE = dict(
+    sinogram_path="synthetic", output_dir=".", output_file="reconstruction.img",
+    FileType=0, FileSubType=0, RawOffset=0,
+    acquisition_fov=50.0, focal_spot_radius=59.5, source_detector_distance=108.56,
+    anode_angle=7.0, axial_focal_spot_shift=0.0375,
+    transaxial_focal_spot_shift=0.042698, wfbp_initialize=0,
+    synthetic=1, synthetic_seed=1,
+)
+
+CONFIGS: dict[str, dict] = {
+    "tiny": dict(
+        Readings=288, n_channels=48, num_views_per_turn_without_ffs=96,
+        axial_detector_spacing=0.22, center_channel_non_ffs=23.5, center_row=3.5,
+        transaxial_detector_spacing=0.228, fan_angle_increment=0.0021,
+        table_feed_per_rotation=0.96, Zffs=0, Phiffs=0, Nrows=8,
+        CollSlicewidth=0.12, recon_fov=4.8, nx=24, ny=24, slice_thickness=0.12,
+        z_start=-1.5, z_end=-0.48, penalty="quadratic", **{"lambda": 0.5},
+        delta=0.02, num_iterations=3, synthetic_cylinder_radius=2.0,
+        synthetic_cylinder_mu=0.193, synthetic_n_ellipsoids=4,
+        synthetic_ellipsoid_radius=1.4, synthetic_axis_min=0.2,
+        synthetic_axis_max=0.8, synthetic_I0=0.0, synthetic_weights=0,
+    ),
+    "tiny_ffs": dict(
+        Readings=288, n_channels=48, num_views_per_turn_without_ffs=48,
+        axial_detector_spacing=0.22, center_channel_non_ffs=23.25, center_row=3.5,
+        transaxial_detector_spacing=0.228, fan_angle_increment=0.0021,
+        table_feed_per_rotation=0.96, Zffs=0, Phiffs=1, Nrows=8,
+        CollSlicewidth=0.12, recon_fov=4.8, nx=24, ny=24, slice_thickness=0.12,
+        z_start=-1.5, z_end=-0.48, penalty="edge-preserving", **{"lambda": 0.5},
+        delta=0.02, num_iterations=3, synthetic_cylinder_radius=2.0,
+        synthetic_cylinder_mu=0.193, synthetic_n_ellipsoids=4,
+        synthetic_ellipsoid_radius=1.4, synthetic_axis_min=0.2,
+        synthetic_axis_max=0.8, synthetic_I0=1e5, synthetic_weights=1,
+    ),
+    "config0": dict(
+        Readings=2304, n_channels=368, num_views_per_turn_without_ffs=576,
+        axial_detector_spacing=0.218945, center_channel_non_ffs=183.5,
+        center_row=7.5, transaxial_detector_spacing=0.255435,
+        fan_angle_increment=0.0023529, table_feed_per_rotation=1.92, Zffs=0,
+        Phiffs=0, Nrows=16, CollSlicewidth=0.12, recon_fov=25.6, nx=128,
+        ny=128, slice_thickness=0.12, z_start=-4.95, z_end=-1.05,
+        penalty="quadratic", **{"lambda": 1000.0}, delta=0.02, num_iterations=5,
+        synthetic_cylinder_radius=10.0, synthetic_cylinder_mu=0.193,
+        synthetic_n_ellipsoids=10, synthetic_ellipsoid_radius=8.0,
+        synthetic_axis_min=0.5, synthetic_axis_max=4.0, synthetic_mu_min=0.15,
+        synthetic_mu_max=0.25, synthetic_I0=1e5, synthetic_weights=1,
+    ),
+    "config1": dict(
+        Readings=6912, n_channels=736, num_views_per_turn_without_ffs=1152,
+        axial_detector_spacing=0.109472, center_channel_non_ffs=366.25,
+        center_row=31.5, transaxial_detector_spacing=0.12858,
+        fan_angle_increment=0.0011844, table_feed_per_rotation=3.84, Zffs=0,
+        Phiffs=1, Nrows=64, CollSlicewidth=0.06, recon_fov=35.84, nx=512,
+        ny=512, slice_thickness=0.06, z_start=-7.5, z_end=-1.71,
+        penalty="edge-preserving", **{"lambda": 5000.0}, delta=0.02,
+        num_iterations=10, synthetic_cylinder_radius=16.0,
+        synthetic_cylinder_mu=0.193, synthetic_n_ellipsoids=10,
+        synthetic_ellipsoid_radius=12.8, synthetic_axis_min=0.5,
+        synthetic_axis_max=4.0, synthetic_mu_min=0.15, synthetic_mu_max=0.25,
+        synthetic_n_small=20, synthetic_small_mu_min=0.3,
+        synthetic_small_mu_max=0.5, synthetic_I0=2e5, synthetic_weights=0,
+    ),
+}
+CONFIGS["micro"] = dict(
+    CONFIGS["tiny"], Readings=96, n_channels=16, num_views_per_turn_without_ffs=32,
+    axial_detector_spacing=0.22, center_channel_non_ffs=7.5, center_row=1.5,
+    transaxial_detector_spacing=0.217, fan_angle_increment=0.002,
+    table_feed_per_rotation=0.48, Nrows=4, recon_fov=1.6, nx=8, ny=8,
+    z_start=-0.85, z_end=-0.43, synthetic_cylinder_radius=0.7,
+    synthetic_n_ellipsoids=2, synthetic_ellipsoid_radius=0.4,
+    synthetic_axis_min=0.1, synthetic_axis_max=0.3)
+CONFIGS["config2"] = dict(CONFIGS["config1"], ssm_fp16=1, resync_every=2)
+CONFIGS["config3"] = dict(CONFIGS["config1"], Readings=4608, nx=256, ny=256,
+                          z_start=-4.0, z_end=-2.05)
+CONFIGS["config4"] = dict(CONFIGS["config1"], Readings=13824, z_start=-16.5,
+                          z_end=-1.11, synthetic_I0=1e5, synthetic_lung=4000,
+                          synthetic_weights=1)
+
+
+def prm_text(name: str, **overrides) -> str:
+    d = dict(_BASE)
+    d.update(CONFIGS[name])
+    d.update(overrides)
+    lines = [f"# {name} (generated by ctbb_icd_amd.configs)"]
+    for k, v in d.items():
+        lines.append(f"{k}: {v}")
+    return "\n".join(lines) + "\n"
+
+
+def write_prm(name: str, path: str | Path, **overrides) -> Path:
+    path = Path(path)
+    path.write_text(prm_text(name, **overrides))
+    return path

@@ -1,0 +1,119 @@
+// common.hpp -- internal definitions shared by the HIP translation units of
+// libctbb_icd.so: error slot, HIP checks, device-resident handle layouts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <ostream>
+#include <string>
+#include <vector>
+
+#include "../../include/ctbb_icd.h"
+
+namespace ctbb {
+
+// Thread-local last-error message (ctbb_last_error).
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+// Report stream (stdout, or a sink when ctbb_set_verbose(0)).
+std::ostream &out();
+
+#define CTBB_HIP(call)                                                        \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess)                                                 \
+            return ::ctbb::fail(CTBB_E_HIP,                                   \
+                                std::string(#call) + ": " +                   \
+                                    hipGetErrorString(e_));                   \
+    } while (0)
+
+#define CTBB_TRY(call)                                                        \
+    do {                                                                      \
+        int rc_ = (call);                                                     \
+        if (rc_ != CTBB_OK)                                                   \
+            return rc_;                                                       \
+    } while (0)
+
+// Device allocation that records a clear message on failure.
+int dev_alloc(void **p, size_t bytes, const char *what);
+
+template <typename T>
+int dev_alloc_n(T **p, size_t n, const char *what)
+{
+    return dev_alloc(reinterpret_cast<void **>(p), n * sizeof(T), what);
+}
+
+// Geometry checks shared by every entry point.
+int check_geom(const ctbb_geom *g);
+
+inline int64_t nffs_of(const ctbb_geom *g)
+{
+    return (int64_t(1) << g->zffs) * (int64_t(1) << g->phiffs);
+}
+
+inline int64_t data_size_of(const ctbb_geom *g)
+{
+    return g->readings * g->n_channels * g->n_rows;
+}
+
+// Sample offset of slice k: (central view - n_views_sm/2) * nch * nr, with
+// the central view aligned down to a multiple of the FFS count
+// (src/icd_iteration.cpp:88-96).
+inline int64_t slice_offset(const ctbb_geom *g, int64_t central)
+{
+    int64_t n = nffs_of(g);
+    if (g->zffs || g->phiffs)
+        central -= central % n;
+    return (central - g->n_views_sm / 2) * g->n_channels * g->n_rows;
+}
+
+}  // namespace ctbb
+
+// Stored system matrix, device resident, compressed sparse column.
+struct ctbb_ssm {
+    ctbb_geom g;
+    int64_t ncol = 0;
+    int64_t nnz = 0;
+    int precision = CTBB_SSM_FP32;
+    int64_t *col_ptr = nullptr;   // [ncol+1]
+    int32_t *idx = nullptr;       // [nnz]
+    float *val = nullptr;         // [nnz]   (FP32 matrices)
+    __half *val16 = nullptr;      // [nnz]   (FP16 matrices)
+    float *col_scale = nullptr;   // [ncol]  (FP16 matrices)
+    int32_t *view_off = nullptr;  // [ncol*n_views_sm] entry offset of each
+                                  // view inside its column (or null)
+    double build_ms = 0.0;
+};
+
+// Reconstruction context.
+struct ctbb_recon {
+    const ctbb_ssm *m = nullptr;
+    ctbb_geom g;
+    ctbb_icd_opts opts;
+    int64_t data_size = 0;
+    int64_t nvox = 0;
+    float *y = nullptr;         // measured line integrals
+    float *w = nullptr;         // weights or null
+    float *e = nullptr;         // running error sinogram y - A x (FP32)
+    double *yhat = nullptr;     // FP64 scratch for resync / cost
+    float *x = nullptr;         // volume, rotated frame
+    int64_t *slice_off = nullptr;  // [nz] sample offsets
+    int32_t *cols = nullptr;    // FOV columns, grouped by batch
+    int64_t *batch_ptr = nullptr;  // [nbatch+1] offsets into cols
+    float *du = nullptr;        // per-step voxel updates
+    int64_t ncols_fov = 0;
+    int64_t nbatch = 0;
+    int64_t max_batch = 0;
+    int64_t tile = 0;
+    std::vector<int64_t> h_batch_ptr;
+    int64_t visits_per_sweep = 0;
+    int64_t ssm_fov_entries = 0;
+    int64_t spacing = 0;
+    double *partials = nullptr; // reduction partials
+    int iter_done = 0;
+    hipStream_t stream = nullptr;
+    std::vector<int64_t> h_slice_index;
+};

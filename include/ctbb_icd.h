@@ -1,0 +1,235 @@
+/*
+ * ctbb_icd.h -- C ABI of the MI355X (gfx950) reconstruction hot path.
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every entry point is
+ * synchronous, returns 0 on success and a negative CTBB_E* code on failure;
+ * ctbb_last_error() then holds a message (thread-local).  The library never
+ * frees caller memory.  Device-resident objects (stored system matrix,
+ * reconstruction context) are opaque handles released explicitly.
+ *
+ * Each entry point names the reference function it stands in for
+ * (paths relative to the CTBB_ICD source tree).
+ */
+#ifndef CTBB_ICD_H
+#define CTBB_ICD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define CTBB_OK            0
+#define CTBB_E_ARG        -1   /* invalid argument / inconsistent sizes   */
+#define CTBB_E_HIP        -2   /* HIP runtime error                       */
+#define CTBB_E_NOMEM      -3   /* device allocation failed                */
+#define CTBB_E_IO         -4   /* file could not be read / written        */
+#define CTBB_E_UNSUPPORTED -5  /* geometry branch not implemented         */
+#define CTBB_E_STATE      -6   /* call made in the wrong state            */
+
+/* ---- geometry --------------------------------------------------------
+ * POD mirror of the numeric fields of `struct recon_params`
+ * (src/recon_structs.h:25-114) that the hot path reads, after the derived
+ * block of configure_recon_params (src/setup.cpp:183-237) and the slice
+ * table of load_raw (src/setup.cpp:349-409) have run.  Units: cm, radians.
+ */
+typedef struct ctbb_geom {
+    int64_t nx, ny, nz;            /* num_voxels_x/y/z                      */
+    int64_t n_channels;            /* n_channels                            */
+    int64_t n_rows;                /* Nrows_projection                      */
+    int64_t readings;              /* Readings (views in the sinogram)      */
+    int64_t n_views_sm;            /* num_views_for_system_matrix           */
+    int64_t zffs, phiffs;          /* Zffs, Phiffs                          */
+    int64_t table_direction;       /* table_direction (-1 or +1)            */
+    double tube_angle_increment;   /* radians per view                      */
+    double tube_z_increment;       /* cm per view                           */
+    double focal_spot_radius;      /* R                                     */
+    double source_detector_distance; /* D                                   */
+    double center_channel_non_ffs;
+    double center_row;
+    double transaxial_detector_spacing;
+    double axial_detector_spacing;
+    double fan_angle_increment;    /* radians                               */
+    double anode_angle;            /* degrees                               */
+    double coll_slicewidth;        /* CollSlicewidth                        */
+    double center_voxel_x, center_voxel_y, center_voxel_z;
+    double voxel_size_x, voxel_size_y, voxel_size_z;
+    double acquisition_fov;        /* cm; ICD updates only x^2+y^2 < (fov/2)^2 */
+} ctbb_geom;
+
+/* ---- library ---------------------------------------------------------- */
+const char *ctbb_last_error(void);
+/* Select the HIP device used by every later call from this thread. */
+int ctbb_init(int device);
+/* Version string of this build. */
+const char *ctbb_version(void);
+
+/* ---- stored system matrix (SSM) --------------------------------------
+ * Replaces generate_system_matrix (src/generate_system_matrix.h:25,
+ * src/generate_system_matrix.cpp:54-209) and generate_system_matrix_ffs
+ * (src/generate_system_matrix_ffs.h:25, src/generate_system_matrix_ffs.cpp:
+ * 60-290, in-plane flying focal spot only).  The matrix is built on the
+ * device in compressed-sparse-column form: column c = ix + nx*iy holds
+ * strictly ascending ray indices q = row + n_rows*(channel + n_channels*view)
+ * over SSM-relative views [0, n_views_sm), with FP32 values (explicit zeros
+ * kept), exactly the entry set the reference pushes for that column.
+ */
+typedef struct ctbb_ssm ctbb_ssm;
+
+#define CTBB_SSM_FP32 0
+#define CTBB_SSM_FP16 1   /* FP16 values with one FP32 scale per column  */
+
+int ctbb_ssm_build(const ctbb_geom *g, ctbb_ssm **out);
+/* Convert the values of a built matrix to FP16 + per-column FP32 scale
+ * (frees the FP32 values). */
+int ctbb_ssm_to_fp16(ctbb_ssm *m);
+int ctbb_ssm_free(ctbb_ssm *m);
+/* nnz: stored entries; precision: CTBB_SSM_*; build_ms: device time of the
+ * build kernels (count + scan + fill). */
+int ctbb_ssm_info(const ctbb_ssm *m, int64_t *nnz, int *precision,
+                  double *build_ms);
+/* Copy the matrix to host arrays: col_ptr[nx*ny+1], idx[nnz], val[nnz]
+ * (FP16 matrices are expanded to FP32 as scale*h). */
+int ctbb_ssm_download(const ctbb_ssm *m, int64_t *col_ptr, int32_t *idx,
+                      float *val);
+/* Copy the columns [c0, c1) only: col_ptr[c1-c0+1] relative to the first of
+ * them, idx/val sized col_ptr[c1-c0]; pass idx = val = NULL to size them. */
+int ctbb_ssm_download_cols(const ctbb_ssm *m, int64_t c0, int64_t c1,
+                           int64_t *col_ptr, int32_t *idx, float *val);
+/* Upload a matrix produced elsewhere (e.g. read from a matrix.bin file,
+ * src/generate_system_matrix.cpp:212-303 format). */
+int ctbb_ssm_upload(const ctbb_geom *g, const int64_t *col_ptr,
+                    const int32_t *idx, const float *val, ctbb_ssm **out);
+/* Write / read the reference's matrix.bin layout: per column a uint64 nnz
+ * followed by nnz x {int32 index, float32 value}
+ * (src/generate_system_matrix.cpp:212-246; read at src/icd_iteration.cpp:
+ * 164-179). */
+int ctbb_ssm_write_file(const ctbb_ssm *m, const char *path);
+int ctbb_ssm_read_file(const ctbb_geom *g, const char *path, ctbb_ssm **out);
+/* Smallest slice spacing s >= 2 such that any two slices whose indices are
+ * congruent mod s have disjoint sinogram footprints (exact per-view test of
+ * the built matrix) in all but a fraction max_conflict_fraction of the
+ * columns inside the acquisition FOV (0 = in every column).
+ * slice_index[nz] = central view of each slice (load_raw slice table). */
+int ctbb_ssm_group_spacing(const ctbb_ssm *m, const int64_t *slice_index,
+                           int64_t max_spacing, double max_conflict_fraction,
+                           int64_t *spacing);
+
+/* ---- forward projection ------------------------------------------------
+ * yhat[r] = sum over columns c, slices k, entries (i,a) with
+ * r = i + (slice_index[k] - n_views_sm/2)*n_channels*n_rows in range of
+ * x[c + nx*ny*k] * a, in FP64 (src/icd_iteration.cpp:63-116, including the
+ * n_ffs alignment of the slice's central view at :90-94).
+ * x: host FP64 volume [nx*ny*nz]; yhat: host FP64 [readings*n_channels*n_rows].
+ */
+int ctbb_forward_project(const ctbb_ssm *m, const int64_t *slice_index,
+                         const double *x, double *yhat);
+
+/* ---- ICD reconstruction context ----------------------------------------
+ * Replaces icd_iteration (src/icd_iteration.h:23, src/icd_iteration.cpp:
+ * 42-286) with the penalties of src/penalties.cpp:39-154 fused in.
+ */
+typedef struct ctbb_recon ctbb_recon;
+
+#define CTBB_PENALTY_QUADRATIC       0
+#define CTBB_PENALTY_EDGE_PRESERVING 1
+
+
+typedef struct ctbb_icd_opts {
+    int32_t penalty;          /* CTBB_PENALTY_*                          */
+    int32_t neighbours;       /* 8 (reference in-plane stencil) or 26    */
+    double lambda;            /* regulariser weight (prm `lambda`)       */
+    double delta;             /* edge-preserving delta (prm `delta`)     */
+    int64_t group_spacing;    /* s; 0 = ctbb_ssm_group_spacing with conflict
+                                 fraction 0 for single-column batches and
+                                 0.01 otherwise                          */
+    int32_t resync_every;     /* FP64 e = y - A x resync period in
+                                 iterations (0 = never)                  */
+    int32_t tile;             /* batch tile: each step updates, in every
+                                 column of one batch, one slice group
+                                 {j, j+s, j+2s, ...}; a batch holds the
+                                 column at the same local position of every
+                                 tile x tile block of columns.  Inside a
+                                 column the update is Gauss-Seidel (the
+                                 group's footprints are disjoint); across the
+                                 batch's columns it is Jacobi (all gather
+                                 from the error sinogram as it stood at the
+                                 start of the step).  0 = 64; tile >=
+                                 max(nx, ny) gives single-column batches,
+                                 i.e. the reference's raster order. */
+} ctbb_icd_opts;
+
+typedef struct ctbb_icd_stats {
+    int64_t iterations;       /* iterations run by the last call          */
+    int64_t voxel_updates;    /* voxel updates performed                  */
+    int64_t entry_visits;     /* in-range SSM entries gathered+scattered  */
+    double sweep_ms;          /* device time of the ICD sweep kernels     */
+    double resync_ms;         /* device time of FP64 resyncs              */
+    double cost;              /* cost after the last iteration (FP64)     */
+    int64_t sweep_launches;   /* kernel launches of the sweeps (a gather
+                                 and a scatter launch per step)          */
+    int64_t ssm_entries;      /* SSM entries of the FOV columns (each read
+                                 once per column per sweep, algorithmically) */
+    double gather_ms_avg;     /* average gather-launch duration (HIP events
+                                 around sampled steps)                   */
+    double scatter_ms_avg;    /* average scatter-launch duration          */
+} ctbb_icd_stats;
+
+/* raw: measured line integrals y [readings*n_channels*n_rows], reference
+ * layout raw[row + n_rows*channel + n_rows*n_channels*view]
+ * (src/setup.cpp:335-344).  weights: per-sample statistical weights of the
+ * same layout, or NULL (unweighted, as the reference).  slice_index[nz]:
+ * central view of every slice (multiple of the FFS count). */
+int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
+                      const float *weights, const int64_t *slice_index,
+                      const ctbb_icd_opts *opts, ctbb_recon **out);
+int ctbb_recon_free(ctbb_recon *r);
+/* Load a start volume (rotated frame, float32 [nx*ny*nz]) and set the
+ * error sinogram e = y - A x in FP64 (the reference forward-projects only
+ * when wfbp_initialize is set, src/icd_iteration.cpp:63; this always does). */
+int ctbb_recon_set_volume(ctbb_recon *r, const float *volume);
+int ctbb_recon_get_volume(const ctbb_recon *r, float *volume);
+/* Run n ICD iterations (src/icd_iteration.cpp:149-274). */
+int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats);
+/* Recompute e = y - A x in FP64 and store it as FP32. */
+int ctbb_recon_resync(ctbb_recon *r);
+/* Global cost of the current volume in FP64 (fresh A x, not the running e):
+ * quadratic:  0.5 sum w e^2 + 0.5 lambda sum_pairs w_d (x_q - x_{q+d})^2
+ * edge-pres.:     sum w e^2 +     lambda sum_pairs w_d rho(x_q - x_{q+d})
+ * rho(t) = |t| - delta ln(1 + |t|/delta); pairs d in {+1, nx-1, nx, nx+1}
+ * with q+d < nx*ny*nz (8-neighbour) and the matching forward half of the
+ * 26-neighbour stencil. */
+int ctbb_recon_cost(ctbb_recon *r, double *cost);
+/* Copy the running FP32 error sinogram to the host. */
+int ctbb_recon_get_error(const ctbb_recon *r, float *e);
+
+/* ---- slice rotation -----------------------------------------------------
+ * Replaces rotate_slices_rotating2fixed / rotate_slices_fixed2rotating
+ * (src/rotate_slices.h:25,29; src/rotate_slices.cpp:48-154 with interp2 of
+ * src/interp.h:41-76).  direction = +1: rotating -> fixed (angle
+ * -tube_angle[slice_index[k]] in radians), -1: fixed -> rotating.
+ * tube_angles in degrees [readings].  volume: host float32 [nx*ny*nz],
+ * rotated in place.  Requires nx == ny (the reference indexes the output
+ * with ny as the x stride, src/rotate_slices.cpp:128). */
+int ctbb_rotate_slices(const ctbb_geom *g, const double *tube_angles,
+                       const int64_t *slice_index, float *volume,
+                       int direction);
+
+/* ---- penalty steps (for checking the fused device penalty) -------------
+ * Evaluates the per-voxel penalty step of src/penalties.cpp:61-154 on the
+ * device for n voxels q[i] of volume x (FP64 [nx*ny*nz]) with data terms
+ * alpha[i], beta[i]; writes du[i] (before the positivity clamp).
+ * solver 0: the reference's bisection, one thread per voxel, FP64 x;
+ * solver 1: the sweep kernel's solver (one wavefront per voxel, 65-section
+ * for the edge-preserving step) on x rounded to FP32, as the sweep sees it. */
+int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *opts,
+                       int solver, const double *x, int64_t n,
+                       const int64_t *q, const double *alpha,
+                       const double *beta, double *du);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CTBB_ICD_H */

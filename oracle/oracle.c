@@ -1,0 +1,598 @@
+/*
+ * oracle.c -- CPU FP64 restatement of the CTBB_ICD reconstruction hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Compiled with -ffp-contract=off
+ * so that every FP64 expression rounds exactly as written, like the
+ * reference binary (x86-64 baseline, -O0, no FMA: src/makefile:12).
+ *
+ * What pins each function (tests/test_oracle_*.py):
+ *   orc_ssm_count/fill   closed forms (partition of unity per ray step,
+ *                        A*1 along a ray), brute-force single-ray re-evaluation,
+ *                        analytic line integral of a smooth object,
+ *                        symmetric central-view cases.  uBLAS expression
+ *                        semantics (element-wise order, norm_2 as the plain
+ *                        sqrt of the sum of squares) are restated, not run:
+ *                        parity with uBLAS itself is unpinned (Boost absent).
+ *   orc_forward_project  dense brute-force A x (numpy) on tiny inputs.
+ *   orc_quadratic        reference src/penalties.cpp compiled from source
+ *   orc_edge_preserving  (oracle/_ref), bit-exact; closed forms.
+ *   orc_icd              dense Gauss-Seidel (numpy) with lambda=0; NNLS limit;
+ *                        single-voxel du = beta/alpha; cost non-increasing.
+ *   orc_cost             gradient of the cost equals the update optimality
+ *                        residual; finite differences.
+ *   orc_rotate_slices    reference src/rotate_slices.cpp + src/interp.h
+ *                        compiled from source (oracle/_ref), bit-exact.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_SQRT2 1.41421356237309504880168872420969807
+#define ORC_PI_ROT 3.141592653589793238   /* src/rotate_slices.cpp:29 */
+#define ORC_PI_FFS 3.14159265359          /* src/generate_system_matrix_ffs.cpp:41 */
+
+/* ------------------------------------------------------------------------ */
+/* Ray geometry of one SSM view (src/generate_system_matrix.cpp:85-100,      */
+/* src/generate_system_matrix_ffs.cpp:112-134, 242-261).                     */
+typedef struct {
+    double ew[3], eu[3], ez[3];
+    double s[3];      /* source position used by the ray loops (FFS-shifted) */
+    double off[3];    /* focal-spot offset (zero without FFS)                */
+} view_geom;
+
+static void orc_view(const ctbb_geom *g, int i, int ffs, double da,
+                     view_geom *v)
+{
+    double n_sm = (double)g->n_views_sm;
+    double tube_angle = (i - 0.5 * n_sm) * g->tube_angle_increment;
+    double cw = cos(tube_angle), sw = sin(tube_angle);
+    v->ew[0] = cw;   v->ew[1] = sw;  v->ew[2] = 0.0;
+    v->eu[0] = -sw;  v->eu[1] = cw;  v->eu[2] = 0.0;
+    v->ez[0] = 0.0;  v->ez[1] = 0.0; v->ez[2] = 1.0;
+    double zs = (double)g->table_direction * ((double)i - 0.5 * n_sm)
+                * g->tube_z_increment;
+    double base[3];
+    for (int e = 0; e < 3; e++)
+        base[e] = g->focal_spot_radius * v->ew[e] + zs * v->ez[e];
+    for (int e = 0; e < 3; e++) {
+        if (ffs)
+            v->off[e] = (i % 2 == 0) ? da * v->eu[e] : (-da) * v->eu[e];
+        else
+            v->off[e] = 0.0;
+    }
+    for (int e = 0; e < 3; e++)
+        v->s[e] = ffs ? base[e] + v->off[e] : base[e];
+}
+
+/* Direction of ray (channel j, row k) for view geometry v. */
+static void orc_dir(const ctbb_geom *g, const view_geom *v, int ffs, int j,
+                    int k, double d[3])
+{
+    double D = g->source_detector_distance;
+    double gamma;
+    if (ffs)
+        gamma = (j - g->center_channel_non_ffs) * g->fan_angle_increment;
+    else
+        gamma = ((j - g->center_channel_non_ffs)
+                 * g->transaxial_detector_spacing) / D;
+    double cg = cos(gamma), sg = sin(gamma);
+    double ap = (k - g->center_row) * g->axial_detector_spacing;
+    double A = -D * cg, B = D * sg;
+    double t[3];
+    for (int e = 0; e < 3; e++) {
+        t[e] = (A * v->ew[e] - B * v->eu[e]) - ap * v->ez[e];
+        if (ffs)
+            t[e] = t[e] - v->off[e];
+    }
+    double nrm;
+    if (ffs) {
+        double acc = 0.0;
+        for (int e = 0; e < 3; e++) {
+            double u = fabs(t[e]);
+            acc += u * u;
+        }
+        nrm = sqrt(acc);
+    } else {
+        nrm = sqrt(ap * ap + D * D);
+    }
+    for (int e = 0; e < 3; e++)
+        d[e] = t[e] / nrm;
+}
+
+typedef void (*emit_fn)(void *ctx, int64_t col, int32_t q, float val);
+
+/* The ray loop of src/generate_system_matrix.cpp:80-186 (and its FFS twin
+ * src/generate_system_matrix_ffs.cpp:104-222): emits (column, q, value) in
+ * ascending q. */
+static int orc_ssm_rays(const ctbb_geom *g, emit_fn emit, void *ctx)
+{
+    int ffs;
+    if (g->zffs == 0 && g->phiffs == 0)
+        ffs = 0;
+    else if (g->zffs == 0 && g->phiffs != 0)
+        ffs = 1;
+    else
+        return -1; /* z flying focal spot: not restated */
+    double D = g->source_detector_distance, R = g->focal_spot_radius;
+    double da = D * R * sin(g->fan_angle_increment) / (4.0 * (D - R));
+    int nx = (int)g->nx, ny = (int)g->ny;
+    for (int i = 0; i < (int)g->n_views_sm; i++) {
+        view_geom v;
+        orc_view(g, i, ffs, da, &v);
+        for (int j = 0; j < (int)g->n_channels; j++) {
+            for (int k = 0; k < (int)g->n_rows; k++) {
+                double d[3];
+                orc_dir(g, &v, ffs, j, k, d);
+                int32_t q = (int32_t)(k + g->n_rows * j
+                                      + g->n_rows * g->n_channels * i);
+                double aax = fabs(d[0]), aay = fabs(d[1]);
+                if (aax > aay) {
+                    for (int ix = 0; ix < nx; ix++) {
+                        double x = (ix - g->center_voxel_x) * g->voxel_size_x;
+                        double y_hat = v.s[1] + d[1] / d[0] * (x - v.s[0]);
+                        double j_hat = y_hat / g->voxel_size_y + g->center_voxel_y;
+                        int jy = (int)floor(j_hat);
+                        double z_hat = v.s[2] + d[2] / d[0] * (x - v.s[0]);
+                        double k_hat = z_hat / g->voxel_size_z + g->center_voxel_z;
+                        int kz = (int)floor(k_hat);
+                        if (jy >= 0 && jy <= ny - 2) {
+                            double scale = g->voxel_size_x / aax;
+                            double tri;
+                            if (kz == -1)
+                                tri = scale * (k_hat - kz);
+                            else if (kz == 0)
+                                tri = scale * (1 - (k_hat - kz));
+                            else
+                                continue;
+                            emit(ctx, ix + (int64_t)nx * jy, q,
+                                 (float)(tri * (1 - (j_hat - jy))));
+                            emit(ctx, ix + (int64_t)nx * (jy + 1), q,
+                                 (float)(tri * (j_hat - jy)));
+                        }
+                    }
+                } else {
+                    for (int iy = 0; iy < ny; iy++) {
+                        double y = (iy - g->center_voxel_y) * g->voxel_size_y;
+                        double x_hat = v.s[0] + d[0] / d[1] * (y - v.s[1]);
+                        double i_hat = x_hat / g->voxel_size_x + g->center_voxel_x;
+                        int ix = (int)floor(i_hat);
+                        double z_hat = v.s[2] + d[2] / d[1] * (y - v.s[1]);
+                        double k_hat = z_hat / g->voxel_size_z + g->center_voxel_z;
+                        int kz = (int)floor(k_hat);
+                        if (ix >= 0 && ix <= nx - 2) {
+                            double scale = g->voxel_size_y / aay;
+                            double tri;
+                            if (kz == -1)
+                                tri = scale * (k_hat - kz);
+                            else if (kz == 0)
+                                tri = scale * (1 - (k_hat - kz));
+                            else
+                                continue;
+                            emit(ctx, ix + (int64_t)nx * iy, q,
+                                 (float)(tri * (1 - (i_hat - ix))));
+                            emit(ctx, (ix + 1) + (int64_t)nx * iy, q,
+                                 (float)(tri * (i_hat - ix)));
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+static void emit_count(void *ctx, int64_t col, int32_t q, float val)
+{
+    (void)q; (void)val;
+    ((int64_t *)ctx)[col] += 1;
+}
+
+typedef struct {
+    int64_t *cursor;
+    int32_t *idx;
+    float *val;
+} fill_ctx;
+
+static void emit_fill(void *ctx, int64_t col, int32_t q, float val)
+{
+    fill_ctx *f = (fill_ctx *)ctx;
+    int64_t p = f->cursor[col]++;
+    f->idx[p] = q;
+    f->val[p] = val;
+}
+
+int64_t orc_ssm_count(const ctbb_geom *g, int64_t *col_cnt)
+{
+    int64_t ncol = g->nx * g->ny;
+    memset(col_cnt, 0, sizeof(int64_t) * ncol);
+    if (orc_ssm_rays(g, emit_count, col_cnt) != 0)
+        return -1;
+    int64_t tot = 0;
+    for (int64_t c = 0; c < ncol; c++)
+        tot += col_cnt[c];
+    return tot;
+}
+
+int orc_ssm_fill(const ctbb_geom *g, const int64_t *col_ptr, int32_t *idx,
+                 float *val)
+{
+    int64_t ncol = g->nx * g->ny;
+    fill_ctx f;
+    f.cursor = (int64_t *)malloc(sizeof(int64_t) * ncol);
+    if (!f.cursor)
+        return -1;
+    memcpy(f.cursor, col_ptr, sizeof(int64_t) * ncol);
+    f.idx = idx;
+    f.val = val;
+    int rc = orc_ssm_rays(g, emit_fill, &f);
+    free(f.cursor);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Slice offset in samples: src/icd_iteration.cpp:88-96 (forward projection) */
+/* and :189-196 (sweep); the n_ffs alignment is a no-op for slice tables     */
+/* built by src/setup.cpp:394-406 (central views step by n_ffs).             */
+static int64_t orc_slice_offset(const ctbb_geom *g, int64_t central_idx)
+{
+    int64_t nffs = ((int64_t)1 << g->zffs) * ((int64_t)1 << g->phiffs);
+    if (g->zffs || g->phiffs)
+        central_idx -= central_idx % nffs;
+    return (central_idx - g->n_views_sm / 2) * g->n_channels * g->n_rows;
+}
+
+void orc_forward_project(const ctbb_geom *g, const int64_t *col_ptr,
+                         const int32_t *idx, const float *val,
+                         const int64_t *slice_index, const double *x,
+                         double *yhat)
+{
+    int64_t data_size = g->readings * g->n_channels * g->n_rows;
+    int64_t nxy = g->nx * g->ny;
+    memset(yhat, 0, sizeof(double) * data_size);
+    for (int64_t c = 0; c < nxy; c++) {
+        for (int64_t k = 0; k < g->nz; k++) {
+            int64_t off = orc_slice_offset(g, slice_index[k]);
+            double xv = x[c + nxy * k];
+            for (int64_t p = col_ptr[c]; p < col_ptr[c + 1]; p++) {
+                int64_t r = idx[p] + off;
+                if (r > -1 && r < data_size)
+                    yhat[r] = yhat[r] + xv * val[p];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Penalties: src/penalties.cpp:39-154.                                       */
+static const double *orc_weights8(double *W)
+{
+    static double w[8];
+    w[0] = 1.0 / ORC_SQRT2; w[1] = 1.0; w[2] = 1.0 / ORC_SQRT2; w[3] = 1.0;
+    w[4] = 1.0; w[5] = 1.0 / ORC_SQRT2; w[6] = 1.0; w[7] = 1.0 / ORC_SQRT2;
+    double s = 0.0;
+    for (int i = 0; i < 8; i++)
+        s += w[i];
+    *W = s;
+    return w;
+}
+
+static void orc_neighbours8(const ctbb_geom *g, int64_t q, int64_t nb[8])
+{
+    int64_t nx = g->nx;
+    nb[0] = q - nx - 1; nb[1] = q - nx; nb[2] = q - nx + 1; nb[3] = q - 1;
+    nb[4] = q + 1; nb[5] = q + nx - 1; nb[6] = q + nx; nb[7] = q + nx + 1;
+}
+
+double orc_quadratic(const ctbb_geom *g, int64_t q, double alpha, double beta,
+                     double lambda, const double *x)
+{
+    double W;
+    const double *w = orc_weights8(&W);
+    int64_t nb[8], n = g->nx * g->ny * g->nz;
+    orc_neighbours8(g, q, nb);
+    double sum = 0.0;
+    for (int i = 0; i < 8; i++)
+        if (nb[i] >= 0 && nb[i] < n)
+            sum += w[i] * (x[nb[i]] - x[q]);
+    return (beta + lambda * sum) / (alpha + lambda * W);
+}
+
+static double orc_ep_func(double du, double alpha, double beta, double delta,
+                          double lambda, const double *x, const double *w,
+                          const int64_t *nb, int64_t q, int64_t n)
+{
+    double sum = 0;
+    for (int i = 0; i < 8; i++) {
+        if (nb[i] >= 0 && nb[i] < n) {
+            double t = (du - (x[nb[i]] - x[q]));
+            if (t >= 0)
+                sum = sum + w[i] * (1.0 - 1.0 / (1.0 + fabs(t / delta)));
+            else
+                sum = sum - w[i] * (1.0 - 1.0 / (1.0 + fabs(t / delta)));
+        }
+    }
+    return 2 * alpha * du - 2 * beta + lambda * sum;
+}
+
+static int orc_sign(double v) { return v < 0 ? -1 : 1; }
+
+double orc_edge_preserving(const ctbb_geom *g, int64_t q, double alpha,
+                           double beta, double lambda, double delta,
+                           const double *x)
+{
+    double W;
+    const double *w = orc_weights8(&W);
+    int64_t nb[8], n = g->nx * g->ny * g->nz;
+    orc_neighbours8(g, q, nb);
+    if (alpha == 0.0)
+        return 0.0;
+    double a = beta / alpha, b = beta / alpha;
+    for (int i = 0; i < 8; i++) {
+        if (nb[i] >= 0 && nb[i] < n) {
+            double diff = (x[nb[i]] - x[q]);
+            if (diff < a) a = diff;
+            if (diff > b) b = diff;
+        }
+    }
+    const int max_iterations = 1000000;
+    const double tol = 0.00000001;
+    double c = (a + b) / 2.0;
+    for (int i = 0; i < max_iterations; i++) {
+        c = (a + b) / 2.0;
+        double f_c = orc_ep_func(c, alpha, beta, delta, lambda, x, w, nb, q, n);
+        double f_a = orc_ep_func(a, alpha, beta, delta, lambda, x, w, nb, q, n);
+        if ((f_c == 0) || (b - a) / 2 < tol)
+            return c;
+        if (orc_sign(f_c) == orc_sign(f_a))
+            a = c;
+        else
+            b = c;
+    }
+    /* The reference falls off the end here (no return); never reached in
+     * FP64 because the bracket halves every step. */
+    return c;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ICD: src/icd_iteration.cpp:149-274 (plus optional weights and the        */
+/* batched z-group order).                                                   */
+/* theta sums and penalty step of voxel (c, k) against the current x, yhat */
+static double orc_voxel_step(const ctbb_geom *g, const int64_t *col_ptr,
+                             const int32_t *idx, const float *val,
+                             const float *raw, const float *weights, int64_t c,
+                             int64_t k, int64_t off, int penalty,
+                             double lambda, double delta, const double *x,
+                             const double *yhat)
+{
+    int64_t data_size = g->readings * g->n_channels * g->n_rows;
+    int64_t q = c + g->nx * g->ny * k;
+    double alpha = 0.0, beta = 0.0;
+    for (int64_t p = col_ptr[c]; p < col_ptr[c + 1]; p++) {
+        int64_t r = idx[p] + off;
+        if (r > -1 && r < data_size) {
+            double a = val[p];
+            double w = weights ? (double)weights[r] : 1.0;
+            alpha += w * a * a;
+            beta += w * a * ((double)raw[r] - yhat[r]);
+        }
+    }
+    double du;
+    if (penalty == CTBB_PENALTY_QUADRATIC)
+        du = orc_quadratic(g, q, alpha, beta, lambda, x);
+    else
+        du = orc_edge_preserving(g, q, alpha, beta, lambda, delta, x);
+    if (du + x[q] < 0)
+        du = -x[q];
+    return du;
+}
+
+static void orc_apply(const ctbb_geom *g, const int64_t *col_ptr,
+                      const int32_t *idx, const float *val, int64_t c,
+                      int64_t k, int64_t off, double du, double *x,
+                      double *yhat)
+{
+    int64_t data_size = g->readings * g->n_channels * g->n_rows;
+    x[c + g->nx * g->ny * k] += du;
+    for (int64_t p = col_ptr[c]; p < col_ptr[c + 1]; p++) {
+        int64_t r = idx[p] + off;
+        if (r > -1 && r < data_size)
+            yhat[r] += du * val[p];
+    }
+}
+
+int64_t orc_icd(const ctbb_geom *g, const int64_t *col_ptr,
+                const int32_t *idx, const float *val, const float *raw,
+                const float *weights, const int64_t *slice_index,
+                int penalty, double lambda, double delta, int64_t spacing,
+                int64_t tile, int n_iter, double *x, double *yhat)
+{
+    int64_t updates = 0;
+    double fov_limit = (g->acquisition_fov / 2.0) * (g->acquisition_fov / 2.0);
+    int64_t *offs = (int64_t *)malloc(sizeof(int64_t) * g->nz);
+    for (int64_t k = 0; k < g->nz; k++)
+        offs[k] = (slice_index[k] - g->n_views_sm / 2)
+                  * g->n_channels * g->n_rows;
+    if (spacing <= 1) {
+        /* the reference's order: y -> x -> z, each update applied at once */
+        for (int n = 0; n < n_iter; n++)
+            for (int64_t j = 0; j < g->ny; j++) {
+                double y = (j - g->center_voxel_y) * g->voxel_size_y;
+                for (int64_t i = 0; i < g->nx; i++) {
+                    double xx = (i - g->center_voxel_x) * g->voxel_size_x;
+                    if (!((xx * xx + y * y) < fov_limit))
+                        continue;
+                    int64_t c = i + g->nx * j;
+                    for (int64_t k = 0; k < g->nz; k++) {
+                        double du = orc_voxel_step(g, col_ptr, idx, val, raw,
+                                                   weights, c, k, offs[k],
+                                                   penalty, lambda, delta, x,
+                                                   yhat);
+                        orc_apply(g, col_ptr, idx, val, c, k, offs[k], du, x,
+                                  yhat);
+                        updates++;
+                    }
+                }
+            }
+        free(offs);
+        return updates;
+    }
+    /* batched z-group order: batches of columns at one local position of
+     * every tile x tile block, raster over local positions; per batch, slice
+     * groups j = 0..s-1; all voxels of a step see the state at its start. */
+    int64_t ntx = (g->nx + tile - 1) / tile, nty = (g->ny + tile - 1) / tile;
+    int64_t *bcols = (int64_t *)malloc(sizeof(int64_t) * ntx * nty);
+    double *dus = (double *)malloc(sizeof(double) * ntx * nty * g->nz);
+    for (int n = 0; n < n_iter; n++) {
+        for (int64_t ly = 0; ly < tile; ly++)
+            for (int64_t lx = 0; lx < tile; lx++) {
+                int64_t nb = 0;
+                for (int64_t ty = 0; ty < nty; ty++)
+                    for (int64_t tx = 0; tx < ntx; tx++) {
+                        int64_t i = tx * tile + lx, j = ty * tile + ly;
+                        if (i >= g->nx || j >= g->ny)
+                            continue;
+                        double xx = (i - g->center_voxel_x) * g->voxel_size_x;
+                        double y = (j - g->center_voxel_y) * g->voxel_size_y;
+                        if ((xx * xx + y * y) < fov_limit)
+                            bcols[nb++] = i + g->nx * j;
+                    }
+                if (nb == 0)
+                    continue;
+                for (int64_t jj = 0; jj < spacing; jj++) {
+                    int64_t t = 0;
+                    for (int64_t b = 0; b < nb; b++)
+                        for (int64_t k = jj; k < g->nz; k += spacing)
+                            dus[t++] = orc_voxel_step(g, col_ptr, idx, val, raw,
+                                                      weights, bcols[b], k,
+                                                      offs[k], penalty, lambda,
+                                                      delta, x, yhat);
+                    t = 0;
+                    for (int64_t b = 0; b < nb; b++)
+                        for (int64_t k = jj; k < g->nz; k += spacing) {
+                            orc_apply(g, col_ptr, idx, val, bcols[b], k, offs[k],
+                                      dus[t++], x, yhat);
+                            updates++;
+                        }
+                }
+            }
+    }
+    free(bcols);
+    free(dus);
+    free(offs);
+    return updates;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cost (no reference counterpart; consistent with the update rules of      */
+/* src/penalties.cpp:84 (quadratic) and :141-154 (edge-preserving)).        */
+double orc_cost(const ctbb_geom *g, const int64_t *col_ptr,
+                const int32_t *idx, const float *val, const float *raw,
+                const float *weights, const int64_t *slice_index,
+                int penalty, double lambda, double delta, const double *x)
+{
+    int64_t data_size = g->readings * g->n_channels * g->n_rows;
+    double *yhat = (double *)malloc(sizeof(double) * data_size);
+    orc_forward_project(g, col_ptr, idx, val, slice_index, x, yhat);
+    double data = 0.0;
+    for (int64_t r = 0; r < data_size; r++) {
+        double e = (double)raw[r] - yhat[r];
+        double w = weights ? (double)weights[r] : 1.0;
+        data += w * e * e;
+    }
+    free(yhat);
+    int64_t n = g->nx * g->ny * g->nz, nx = g->nx;
+    int64_t dd[4] = {1, nx - 1, nx, nx + 1};
+    double wd[4] = {1.0, 1.0 / ORC_SQRT2, 1.0, 1.0 / ORC_SQRT2};
+    double pen = 0.0;
+    for (int64_t q = 0; q < n; q++) {
+        for (int t = 0; t < 4; t++) {
+            if (q + dd[t] >= n)
+                continue;
+            double diff = x[q] - x[q + dd[t]];
+            if (penalty == CTBB_PENALTY_QUADRATIC) {
+                pen += wd[t] * diff * diff;
+            } else {
+                double a = fabs(diff);
+                pen += wd[t] * (a - delta * log1p(a / delta));
+            }
+        }
+    }
+    if (penalty == CTBB_PENALTY_QUADRATIC)
+        return 0.5 * data + 0.5 * lambda * pen;
+    return data + lambda * pen;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Slice rotation: src/rotate_slices.cpp:48-154, interp2 src/interp.h:41-76. */
+static float orc_interp2(const float *a, int d1, int d2, float i1, float i2)
+{
+    float val;
+    if (i1 > (float)(d1 - 2) || (i1 < 0) || i2 > (float)(d2 - 2) || (i2 < 0)) {
+        val = 0.0f;
+    } else {
+        float v = i1 - floorf(i1);
+        float w = i2 - floorf(i2);
+        int a1 = (int)floorf(i2) * d1 + (int)floorf(i1);
+        int a2 = (int)floorf(i2) * d1 + (int)floorf(i1) + 1;
+        int a3 = ((int)floorf(i2) + 1) * d1 + (int)floorf(i1);
+        int a4 = ((int)floorf(i2) + 1) * d1 + (int)floorf(i1) + 1;
+        val = a[a1] * (1.0f - v) * (1.0f - w) + a[a2] * v * (1.0f - w)
+              + a[a3] * (1.0f - v) * w + a[a4] * v * w;
+    }
+    return val;
+}
+
+static void orc_rotate_slice(const ctbb_geom *g, const float *in, float *out,
+                             double angle)
+{
+    int nx = (int)g->nx, ny = (int)g->ny;
+    for (int xi = 0; xi < nx; xi++) {
+        for (int yi = 0; yi < ny; yi++) {
+            int64_t out_idx = xi + (int64_t)yi * ny;
+            double x = xi - g->center_voxel_x;
+            double y = yi - g->center_voxel_y;
+            float xp = (float)(cos(angle) * x - sin(angle) * y);
+            float yp = (float)(sin(angle) * x + cos(angle) * y);
+            out[out_idx] = orc_interp2(in, nx, ny,
+                                       (float)(xp + g->center_voxel_x),
+                                       (float)(yp + g->center_voxel_y));
+        }
+    }
+}
+
+void orc_rotate_slices(const ctbb_geom *g, const double *tube_angles,
+                       const int64_t *slice_index, float *volume,
+                       int direction)
+{
+    int64_t npix = g->nx * g->ny;
+    float *tmp = (float *)malloc(sizeof(float) * npix);
+    for (int64_t k = 0; k < g->nz; k++) {
+        double angle = -(tube_angles[slice_index[k]]) * 2.0 * ORC_PI_ROT / 360.0;
+        if (direction < 0)
+            angle = -angle;
+        memcpy(tmp, volume + k * npix, sizeof(float) * npix);
+        orc_rotate_slice(g, tmp, volume + k * npix, angle);
+    }
+    free(tmp);
+}
+
+/* ------------------------------------------------------------------------ */
+int orc_column_overlaps(const ctbb_geom *g, const int64_t *col_ptr,
+                        const int32_t *idx, int64_t c, int64_t dv)
+{
+    int64_t shift = dv * g->n_channels * g->n_rows;
+    int64_t a = col_ptr[c], b = col_ptr[c], end = col_ptr[c + 1];
+    /* S intersect (S + shift): walk idx[a] against idx[b] + shift */
+    while (a < end && b < end) {
+        int64_t va = idx[a], vb = (int64_t)idx[b] + shift;
+        if (va == vb)
+            return 1;
+        if (va < vb)
+            a++;
+        else
+            b++;
+    }
+    return 0;
+}

@@ -1,0 +1,43 @@
+"""The C-ABI library loads (no GPU needed) and exports every declared entry."""
+import re
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        txt = h.read_text()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"\b(ctbb_[a-z0-9_]+)\s*\(", txt):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol(ct):
+    import ctypes
+    L = ct.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = []
+    for n in names:
+        try:
+            getattr(L, n)
+        except AttributeError:
+            missing.append(n)
+    assert not missing, missing
+
+
+def test_version_and_last_error(ct):
+    assert b"gfx950" in ct.lib().ctbb_version()
+    # a host-side error path: a missing prm file
+    rc = ct.lib().ctbb_configure(b"/nonexistent/prm.yaml", None, None, None, None, None, None)
+    assert rc != 0
+    assert ct.lib().ctbb_last_error()
+
+
+def test_device_entry_points_reject_null(ct):
+    import ctypes
+    assert ct.lib().ctbb_ssm_build(None, None) == -1
+    assert ct.lib().ctbb_recon_iterate(None, 1, None) == -1
