@@ -51,14 +51,10 @@ def parse():
     return ap.parse_args()
 
 
-def copy_probe(mib: int) -> None:
-    import torch
-    n = mib * 1024 * 1024 // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    print(f"[bench] copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr)
+def copy_probe(ct, mib: int) -> None:
+    ms = ct.copy_probe(mib * 1024 * 1024)
+    print(f"[bench] copy probe (ctbb_copy_kernel): {mib} MiB read + {mib} MiB "
+          f"written in {ms:.3f} ms ({2 * mib / 1024 / ms * 1e3:.0f} GiB/s)", file=sys.stderr)
 
 
 def cpu_baseline(scan, ssm, raw, w, target_s: float, threads: int):
@@ -104,7 +100,7 @@ def main():
     ct.set_verbose(False)
     ct.init(0)
     if args.copy_probe_mib:
-        copy_probe(args.copy_probe_mib)
+        copy_probe(ct, args.copy_probe_mib)
     tmp = tempfile.TemporaryDirectory()
     over = {"ssm_fp16": 1} if args.fp16 else {}
     scan = ct.Scan(configs.write_prm(args.config, Path(tmp.name) / "bench.yaml", **over))
@@ -169,7 +165,7 @@ def main():
                         f"{'FP16' if args.fp16 else 'FP32'} SSM, "
                         f"{'edge-preserving' if scan.run.penalty else 'quadratic'} penalty",
             "voxels": g.nvox, "ssm_nnz": nnz, "slice_spacing": spacing,
-            "tile": args.tile or 64,
+            "tile": args.tile or 8,
             "parallelism": "1 GPU",
         },
         "roofline": {

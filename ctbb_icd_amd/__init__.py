@@ -102,6 +102,7 @@ def lib() -> C.CDLL:
         "ctbb_version": ([], C.c_char_p),
         "ctbb_init": ([C.c_int], C.c_int),
         "ctbb_set_verbose": ([C.c_int], None),
+        "ctbb_copy_probe": ([_i64, C.POINTER(_f64)], C.c_int),
         "ctbb_configure": ([C.c_char_p, C.POINTER(Geom), C.POINTER(RunParams), P, P, P, P], C.c_int),
         "ctbb_synthesize": ([C.c_char_p, P, P, C.c_int], C.c_int),
         "ctbb_phantom_volume": ([C.c_char_p, P], C.c_int),
@@ -172,6 +173,10 @@ class Scan:
                                    _p(self.tube_angles), _p(self.table_positions),
                                    _p(self.slice_locations), _p(self.slice_index)))
         self.geom, self.run = g, rp
+        self.tile = 0
+        for line in open(self.prm_path):
+            if line.split(":")[0].strip() == "tile":
+                self.tile = int(line.split(":", 1)[1].split("#")[0])
 
     def synthesize(self, use_gpu: bool = False, weights: bool | None = None):
         g = self.geom
@@ -195,7 +200,7 @@ class Scan:
         o.delta = kw.get("delta", self.run.delta)
         o.group_spacing = kw.get("group_spacing", self.run.group_spacing)
         o.resync_every = kw.get("resync_every", self.run.resync_every)
-        o.tile = kw.get("tile", 0)
+        o.tile = kw.get("tile", self.tile)
         return o
 
 
@@ -362,6 +367,13 @@ def penalty_steps(geom: Geom, opts: IcdOpts, x, q, alpha, beta, solver: int = 0)
     check(lib().ctbb_penalty_steps(C.byref(geom), C.byref(opts), solver, _p(x), len(q), _p(q),
                                    _p(a), _p(b), _p(du)))
     return du
+
+
+def copy_probe(nbytes: int) -> float:
+    """Run the calibration copy kernel over nbytes; returns milliseconds."""
+    ms = C.c_double()
+    check(lib().ctbb_copy_probe(nbytes, C.byref(ms)))
+    return ms.value
 
 
 def init(device: int = 0) -> None:

@@ -67,6 +67,9 @@ CONFIGS: dict[str, dict] = {
         synthetic_n_ellipsoids=10, synthetic_ellipsoid_radius=8.0,
         synthetic_axis_min=0.5, synthetic_axis_max=4.0, synthetic_mu_min=0.15,
         synthetic_mu_max=0.25, synthetic_I0=1e5, synthetic_weights=1,
+        # the weighted data term dominates the penalty here: batches of
+        # columns 16 apart (64 per batch) keep the Jacobi coupling small
+        tile=16,
     ),
     "config1": dict(
         Readings=6912, n_channels=736, num_views_per_turn_without_ffs=1152,

@@ -86,6 +86,10 @@ struct ctbb_ssm {
     int32_t *view_off = nullptr;  // [ncol*n_views_sm] entry offset of each
                                   // view inside its column (or null)
     double build_ms = 0.0;
+    // ray-major copy (CSR over rays), built on first use by the projection
+    int64_t *rm_ptr = nullptr;    // [n_views_sm*n_channels*n_rows + 1]
+    int32_t *rm_col = nullptr;    // [nnz] column of each entry
+    float *rm_val = nullptr;      // [nnz]
 };
 
 // Reconstruction context.
@@ -98,7 +102,6 @@ struct ctbb_recon {
     float *y = nullptr;         // measured line integrals
     float *w = nullptr;         // weights or null
     float *e = nullptr;         // running error sinogram y - A x (FP32)
-    double *yhat = nullptr;     // FP64 scratch for resync / cost
     float *x = nullptr;         // volume, rotated frame
     int64_t *slice_off = nullptr;  // [nz] sample offsets
     int32_t *cols = nullptr;    // FOV columns, grouped by batch
@@ -109,6 +112,16 @@ struct ctbb_recon {
     int64_t max_batch = 0;
     int64_t tile = 0;
     std::vector<int64_t> h_batch_ptr;
+    std::vector<int64_t> slice_off_h;   // [nz] sample offsets (host copy)
+    std::vector<int64_t> h_si_half;     // [nz] central view - n_views_sm/2
+    // transposed batches (sample-driven scatter)
+    uint32_t *bt_ent = nullptr;   // packed slot | row << 14 | channel << 22
+    float *bt_val = nullptr;
+    int32_t *bt_vc = nullptr;     // [nbatch][n_views_sm*n_channels+1]
+    int64_t *bt_off = nullptr;    // [nbatch]
+    int64_t *si_half = nullptr;   // [nz] device copy of h_si_half
+    int32_t *k_lo = nullptr;      // [readings] slices whose view window holds
+    int32_t *k_hi = nullptr;      //   the view: [k_lo, k_hi)
     int64_t visits_per_sweep = 0;
     int64_t ssm_fov_entries = 0;
     int64_t spacing = 0;

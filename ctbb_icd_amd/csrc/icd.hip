@@ -34,6 +34,8 @@
 #include <cstring>
 
 
+using namespace ctbb;
+
 namespace {
 
 constexpr double kSqrt2 = 1.41421356237309504880168872420969807;
@@ -382,91 +384,190 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
             if (du + xq < 0)
                 du = -xq;
             a.x[q] = (float)(xq + du);
-            a.du[t] = (float)du;
+            a.du[v * sa.nc + ci] = (float)du;
         }
     }
 }
 
-// Phase B of a step: every voxel task scatters e -= du a (atomic adds:
-// columns of one batch may share rays where their sinogram traces cross;
-// slices of one group in one column do not).
-template <bool FP16>
-__global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(SweepArgs a,
-                                                                  StepArgs sa)
+// Phase B of a step: sample-driven scatter.  Each workgroup owns the
+// contiguous samples of one data view and kScatterCh channels (all rows) and
+// adds up, in LDS, the updates of every voxel of the step that reaches them:
+// for each slice k of the group, the SSM view i = view - (c_k - n_sm/2) and
+// the batch's transposed entries of that view and those channels (sorted by
+// ray, then column slot) give the rays and their column slots.  A ray's
+// entries are summed by one thread; slices are taken one after the other.
+// The workgroup then writes each changed sample once: no two workgroups
+// touch the same sample, so the scatter is conflict-free and deterministic.
+constexpr int kScatterCh = 32;
+constexpr uint32_t kSlotBits = 14, kRowBits = 8;
+
+__device__ __forceinline__ uint32_t bt_slot(uint32_t p) { return p & ((1u << kSlotBits) - 1); }
+__device__ __forceinline__ uint32_t bt_row(uint32_t p) { return (p >> kSlotBits) & ((1u << kRowBits) - 1); }
+__device__ __forceinline__ uint32_t bt_ch(uint32_t p) { return p >> (kSlotBits + kRowBits); }
+
+struct BtArgs {
+    const uint32_t *ent;     // packed slot | row << 14 | ch << 22
+    const float *val;        // entry values (FP32)
+    const int32_t *vc_idx;   // [nbatch][nsm*nch+1] segment starts per (view, ch)
+    const int64_t *ent_off;  // [nbatch] first entry of each batch
+    const int64_t *si_half;  // [nz] central view - n_sm/2 of each slice
+    int64_t nsm, nch, nr, readings;
+};
+
+__global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
+    SweepArgs a, StepArgs sa, BtArgs bt, int64_t b, int64_t vlo, int64_t ncg)
 {
+    __shared__ float acc[kScatterCh * 256];
     const int tid = threadIdx.x;
-    const int64_t t = blockIdx.x;
-    int64_t ci, v;
-    if (!task_of(t, sa.ng, sa.nc, ci, v))
-        return;
-    const float du = a.du[t];
-    if (du == 0.0f)
-        return;
-    const int64_t col = a.cols[sa.c0 + ci];
-    const int64_t off = a.slice_off[sa.j + a.spacing * v];
-    const int64_t beg = a.col_ptr[col], len = a.col_ptr[col + 1] - beg;
-    const float scale = FP16 ? a.col_scale[col] : 1.0f;
-    const int32_t *__restrict__ idx = a.idx + beg;
-    float *__restrict__ e = a.e;
-    const int64_t ds = a.data_size;
-    for (int64_t base = tid; base < len; base += kSweepBlock * kUnroll) {
-        int64_t r[kUnroll];
-        float dv[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const int64_t o = base + u * kSweepBlock;
-            const int64_t oc = o < len ? o : len - 1;
-            r[u] = (int64_t)idx[oc] + off;
-            dv[u] = -du * ssm_value<FP16>(a, beg + oc, scale);
-            if (!(o < len && r[u] >= 0 && r[u] < ds))
-                r[u] = -1;
+    const int64_t view = vlo + blockIdx.x / ncg;
+    const int64_t c0 = (blockIdx.x % ncg) * kScatterCh;
+    const int64_t c1 = min(c0 + kScatterCh, bt.nch);
+    const int nsamp = (int)((c1 - c0) * bt.nr);
+    for (int i = tid; i < nsamp; i += kSweepBlock)
+        acc[i] = 0.0f;
+    const int64_t nvc = bt.nsm * bt.nch + 1;
+    const int32_t *vci = bt.vc_idx + b * nvc;
+    const uint32_t *ent = bt.ent + bt.ent_off[b];
+    const float *val = bt.val + bt.ent_off[b];
+    bool any = false;
+    for (int64_t v = 0; v < sa.ng; v++) {
+        const int64_t k = sa.j + a.spacing * v;
+        const int64_t i = view - bt.si_half[k];
+        if (i < 0 || i >= bt.nsm)
+            continue;
+        const int64_t lo = vci[i * bt.nch + c0], hi = vci[i * bt.nch + c1];
+        if (lo == hi)
+            continue;
+        any = true;
+        __syncthreads();   // previous slice's LDS adds are complete
+        const float *du = a.du + v * sa.nc;
+        for (int64_t p = lo + tid; p < hi; p += kSweepBlock) {
+            const uint32_t e0 = ent[p];
+            const uint32_t ray = e0 >> kSlotBits;
+            if (p > lo && (ent[p - 1] >> kSlotBits) == ray)
+                continue;   // not the first entry of its ray
+            float sum = 0.0f;
+            for (int64_t pp = p; pp < hi && (ent[pp] >> kSlotBits) == ray; pp++)
+                sum += du[bt_slot(ent[pp])] * val[pp];
+            acc[(int)((bt_ch(e0) - c0) * bt.nr + bt_row(e0))] += sum;
         }
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++)
-            if (r[u] >= 0)
-                atomicAdd(&e[r[u]], dv[u]);
+    }
+    if (!any)
+        return;
+    __syncthreads();
+    float *e = a.e + (view * bt.nch + c0) * bt.nr;
+    for (int i = tid; i < nsamp; i += kSweepBlock) {
+        const float d = acc[i];
+        if (d != 0.0f)
+            e[i] = e[i] - d;
+    }
+}
+
+// Transposed-batch construction: per FOV column its batch and slot.
+__global__ void bt_count_kernel(const int64_t *col_ptr, const int32_t *idx,
+                                const int32_t *cols, int64_t ncols,
+                                const int32_t *col_batch, int64_t nr,
+                                int64_t nvc, uint32_t *counts)
+{
+    for (int64_t i = blockIdx.x; i < ncols; i += gridDim.x) {
+        const int64_t col = cols[i];
+        const int64_t b = col_batch[col];
+        for (int64_t p = col_ptr[col] + threadIdx.x; p < col_ptr[col + 1];
+             p += blockDim.x)
+            atomicAdd(&counts[b * nvc + idx[p] / nr], 1u);
+    }
+}
+
+// Exclusive scan of each batch's (view, channel) counts; one block per batch.
+__global__ __launch_bounds__(1024) void bt_scan_kernel(const uint32_t *counts,
+                                                       int32_t *vc_idx,
+                                                       int64_t *batch_tot,
+                                                       int64_t nvc)
+{
+    __shared__ int64_t part[1024];
+    const int64_t b = blockIdx.x;
+    const uint32_t *c = counts + b * nvc;
+    int32_t *o = vc_idx + b * nvc;
+    const int t = threadIdx.x;
+    const int64_t per = (nvc + 1023) / 1024;
+    const int64_t lo = t * per, hi = min(nvc, lo + per);
+    int64_t s = 0;
+    for (int64_t i = lo; i < hi; i++)
+        s += c[i];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        int64_t run = 0;
+        for (int i = 0; i < 1024; i++) {
+            const int64_t v = part[i];
+            part[i] = run;
+            run += v;
+        }
+        batch_tot[b] = run;
+    }
+    __syncthreads();
+    int64_t run = part[t];
+    for (int64_t i = lo; i < hi; i++) {
+        o[i] = (int32_t)run;
+        run += c[i];
+    }
+}
+
+__global__ void bt_fill_kernel(const int64_t *col_ptr, const int32_t *idx,
+                               const float *val, const __half *val16,
+                               const float *col_scale, const int32_t *cols,
+                               int64_t ncols, const int32_t *col_batch,
+                               const int32_t *col_slot, int64_t nr, int64_t nch,
+                               int64_t nvc, const int32_t *vc_idx,
+                               const int64_t *ent_off, uint32_t *cursor,
+                               uint32_t *ent, float *bval)
+{
+    for (int64_t i = blockIdx.x; i < ncols; i += gridDim.x) {
+        const int64_t col = cols[i];
+        const int64_t b = col_batch[col];
+        const uint32_t slot = (uint32_t)col_slot[col];
+        for (int64_t p = col_ptr[col] + threadIdx.x; p < col_ptr[col + 1];
+             p += blockDim.x) {
+            const int64_t q = idx[p];
+            const int64_t key = q / nr;
+            const uint32_t row = (uint32_t)(q - key * nr);
+            const uint32_t ch = (uint32_t)(key % nch);
+            const int64_t pos = ent_off[b] + vc_idx[b * nvc + key] +
+                                atomicAdd(&cursor[b * nvc + key], 1u);
+            ent[pos] = slot | (row << kSlotBits) | (ch << (kSlotBits + kRowBits));
+            bval[pos] = val16 ? __half2float(val16[p]) * col_scale[col] : val[p];
+        }
+    }
+}
+
+// Sort each (view, channel) segment by (row, slot): a deterministic order.
+__global__ void bt_sort_kernel(const int32_t *vc_idx, const int64_t *ent_off,
+                               int64_t nbatch, int64_t nvc, uint32_t *ent,
+                               float *bval)
+{
+    const int64_t nseg = nbatch * (nvc - 1);
+    for (int64_t sg = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; sg < nseg;
+         sg += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = sg / (nvc - 1), key = sg - b * (nvc - 1);
+        const int64_t lo = ent_off[b] + vc_idx[b * nvc + key];
+        const int64_t hi = ent_off[b] + vc_idx[b * nvc + key + 1];
+        for (int64_t i = lo + 1; i < hi; i++) {
+            const uint32_t ke = ent[i];
+            const float kv = bval[i];
+            int64_t jj = i - 1;
+            while (jj >= lo && ent[jj] > ke) {
+                ent[jj + 1] = ent[jj];
+                bval[jj + 1] = bval[jj];
+                jj--;
+            }
+            ent[jj + 1] = ke;
+            bval[jj + 1] = kv;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Forward projection yhat = A x in FP64 (src/icd_iteration.cpp:63-116).
-
-constexpr int kFwdBlock = 256;
-
-template <typename X, bool FP16>
-__global__ __launch_bounds__(kFwdBlock) void fwd_scatter_kernel(
-    const int64_t *col_ptr, const int32_t *idx, const float *val,
-    const __half *val16, const float *col_scale, const X *x,
-    const int64_t *slice_off, double *yhat, int64_t ncol, int64_t nz,
-    int64_t data_size)
-{
-    for (int64_t col = blockIdx.x; col < ncol; col += gridDim.x) {
-        const int64_t beg = col_ptr[col], end = col_ptr[col + 1];
-        const float scale = FP16 ? col_scale[col] : 1.0f;
-        for (int64_t k = 0; k < nz; k++) {
-            const double xv = (double)x[col + ncol * k];
-            if (xv == 0.0)
-                continue;
-            const int64_t off = slice_off[k];
-            for (int64_t p = beg + threadIdx.x; p < end; p += kFwdBlock) {
-                const int64_t r = (int64_t)idx[p] + off;
-                if (r > -1 && r < data_size) {
-                    const double av =
-                        FP16 ? (double)(__half2float(val16[p]) * scale)
-                             : (double)val[p];
-                    unsafeAtomicAdd(&yhat[r], xv * av);
-                }
-            }
-        }
-    }
-}
-
-__global__ void fill_f64_kernel(double *p, int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = 0.0;
-}
+// Reductions, resync and cost.
 
 constexpr int kRedBlock = 256;
 
@@ -566,6 +667,148 @@ __global__ void inrange_count_kernel(const int64_t *col_ptr,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Ray-major copy of the SSM (CSR over rays q = row + nr*(ch + nch*view)) and
+// the sample-driven FP64 forward projection built on it: every sample r
+// gathers, for each slice k whose view window contains r's view, the entries
+// of ray q = r - slice offset, in a fixed order (no atomics, deterministic).
+
+__global__ void rm_count_kernel(const int64_t *col_ptr, const int32_t *idx,
+                                int64_t ncol, uint32_t *cnt)
+{
+    for (int64_t c = blockIdx.x; c < ncol; c += gridDim.x)
+        for (int64_t p = col_ptr[c] + threadIdx.x; p < col_ptr[c + 1]; p += blockDim.x)
+            atomicAdd(&cnt[idx[p]], 1u);
+}
+
+constexpr int kScanChunk = 65536;
+
+__global__ __launch_bounds__(256) void chunk_sum_kernel(const uint32_t *cnt,
+                                                        int64_t n, int64_t *sums)
+{
+    __shared__ int64_t sh[256];
+    const int64_t b0 = (int64_t)blockIdx.x * kScanChunk;
+    int64_t s = 0;
+    for (int64_t i = b0 + threadIdx.x; i < min(n, b0 + kScanChunk); i += 256)
+        s += cnt[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int i = 0; i < 256; i++)
+            t += sh[i];
+        sums[blockIdx.x] = t;
+    }
+}
+
+// exclusive scan of one chunk with a base offset: 256 threads, each a
+// contiguous run of kScanChunk/256 elements
+__global__ __launch_bounds__(256) void chunk_scan_kernel(const uint32_t *cnt,
+                                                         int64_t n,
+                                                         const int64_t *base,
+                                                         int64_t *out)
+{
+    __shared__ int64_t sh[256];
+    const int64_t b0 = (int64_t)blockIdx.x * kScanChunk;
+    const int64_t per = kScanChunk / 256;
+    const int64_t lo = b0 + threadIdx.x * per, hi = min(n, lo + per);
+    int64_t s = 0;
+    for (int64_t i = lo; i < hi; i++)
+        s += cnt[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = base[blockIdx.x];
+        for (int i = 0; i < 256; i++) {
+            const int64_t v = sh[i];
+            sh[i] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    int64_t run = sh[threadIdx.x];
+    for (int64_t i = lo; i < hi; i++) {
+        out[i] = run;
+        run += cnt[i];
+    }
+}
+
+__global__ void rm_fill_kernel(const int64_t *col_ptr, const int32_t *idx,
+                               const float *val, const __half *val16,
+                               const float *col_scale, int64_t ncol,
+                               unsigned long long *cursor, int32_t *rcol,
+                               float *rval)
+{
+    for (int64_t c = blockIdx.x; c < ncol; c += gridDim.x)
+        for (int64_t p = col_ptr[c] + threadIdx.x; p < col_ptr[c + 1]; p += blockDim.x) {
+            const unsigned long long pos = atomicAdd(&cursor[idx[p]], 1ull);
+            rcol[pos] = (int32_t)c;
+            rval[pos] = val16 ? __half2float(val16[p]) * col_scale[c] : val[p];
+        }
+}
+
+__global__ void rm_sort_kernel(const int64_t *rptr, int64_t nrays, int32_t *rcol,
+                               float *rval)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nrays;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t lo = rptr[q], hi = rptr[q + 1];
+        for (int64_t i = lo + 1; i < hi; i++) {
+            const int32_t kc = rcol[i];
+            const float kv = rval[i];
+            int64_t j = i - 1;
+            while (j >= lo && rcol[j] > kc) {
+                rcol[j + 1] = rcol[j];
+                rval[j + 1] = rval[j];
+                j--;
+            }
+            rcol[j + 1] = kc;
+            rval[j + 1] = kv;
+        }
+    }
+}
+
+// Sample-driven FP64 projection.  MODE 0: yhat[r] = (A x)[r].  MODE 1:
+// e[r] = (float)(y[r] - (A x)[r]) (if e) and block partials of
+// w (y - A x)^2.  k_lo/k_hi[view]: slices whose view window holds the view.
+template <typename X, int MODE>
+__global__ __launch_bounds__(kRedBlock) void project_gather_kernel(
+    const int64_t *rptr, const int32_t *rcol, const float *rval, const X *x,
+    const int64_t *si_half, const int32_t *k_lo, const int32_t *k_hi,
+    int64_t nsm, int64_t per_view, int64_t nxy, int64_t data_size,
+    double *yhat, const float *y, const float *w, float *e, double *partials)
+{
+    __shared__ double sh[kRedBlock / 64];
+    double part = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; r < data_size;
+         r += (int64_t)gridDim.x * kRedBlock) {
+        const int64_t view = r / per_view, rem = r - view * per_view;
+        double acc = 0.0;
+        for (int32_t k = k_lo[view]; k < k_hi[view]; k++) {
+            const int64_t i = view - si_half[k];
+            if (i < 0 || i >= nsm)
+                continue;
+            const int64_t q = i * per_view + rem;
+            const X *xk = x + nxy * k;
+            for (int64_t p = rptr[q]; p < rptr[q + 1]; p++)
+                acc += (double)xk[rcol[p]] * (double)rval[p];
+        }
+        if (MODE == 0) {
+            yhat[r] = acc;
+        } else {
+            const double d = (double)y[r] - acc;
+            if (e)
+                e[r] = (float)d;
+            part += (w ? (double)w[r] : 1.0) * d * d;
+        }
+    }
+    if (MODE == 1) {
+        const double s = block_sum_f64(part, sh);
+        if (threadIdx.x == 0)
+            partials[blockIdx.x] = s;
+    }
+}
+
 int g_num_cu = 0;
 
 int num_cu()
@@ -586,25 +829,130 @@ int red_grid(int64_t n)
                                                         4 * 1024));
 }
 
-// yhat = A x (FP64) on the recon stream.
-template <typename X>
-int launch_forward(const ctbb_ssm *m, const X *x, const int64_t *slice_off,
-                   double *yhat, int64_t data_size, hipStream_t st)
+// Build (once) the ray-major copy of the matrix.
+int ensure_ray_major(const ctbb_ssm *mc)
 {
-    hipLaunchKernelGGL(fill_f64_kernel, dim3(4096), dim3(256), 0, st, yhat,
-                       data_size);
-    CTBB_HIP(hipGetLastError());
-    const int grid = (int)std::min<int64_t>(m->ncol, 65536);
-    if (m->precision == CTBB_SSM_FP16)
-        hipLaunchKernelGGL((fwd_scatter_kernel<X, true>), dim3(grid),
-                           dim3(kFwdBlock), 0, st, m->col_ptr, m->idx, m->val,
-                           m->val16, m->col_scale, x, slice_off, yhat, m->ncol,
-                           m->g.nz, data_size);
-    else
-        hipLaunchKernelGGL((fwd_scatter_kernel<X, false>), dim3(grid),
-                           dim3(kFwdBlock), 0, st, m->col_ptr, m->idx, m->val,
-                           m->val16, m->col_scale, x, slice_off, yhat, m->ncol,
-                           m->g.nz, data_size);
+    ctbb_ssm *m = const_cast<ctbb_ssm *>(mc);
+    if (m->rm_ptr)
+        return CTBB_OK;
+    const ctbb_geom *g = &m->g;
+    const int64_t nrays = g->n_views_sm * g->n_channels * g->n_rows;
+    uint32_t *cnt = nullptr;
+    int64_t *sums = nullptr;
+    unsigned long long *cursor = nullptr;
+    const int64_t nchunk = (nrays + kScanChunk - 1) / kScanChunk;
+    int rc = CTBB_OK;
+    auto cleanup = [&]() {
+        if (cnt) (void)hipFree(cnt);
+        if (sums) (void)hipFree(sums);
+        if (cursor) (void)hipFree(cursor);
+    };
+    if ((rc = dev_alloc_n(&cnt, nrays, "ray counts")) ||
+        (rc = dev_alloc_n(&sums, nchunk, "chunk sums")) ||
+        (rc = dev_alloc_n(&m->rm_ptr, nrays + 1, "ray pointers")) ||
+        (rc = dev_alloc_n(&m->rm_col, std::max<int64_t>(1, m->nnz), "ray-major columns")) ||
+        (rc = dev_alloc_n(&m->rm_val, std::max<int64_t>(1, m->nnz), "ray-major values"))) {
+        cleanup();
+        return rc;
+    }
+    const int gcol = (int)std::min<int64_t>(m->ncol, 65536);
+    hipError_t e = hipMemset(cnt, 0, nrays * 4);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(rm_count_kernel, dim3(gcol), dim3(256), 0, 0, m->col_ptr,
+                           m->idx, m->ncol, cnt);
+        hipLaunchKernelGGL(chunk_sum_kernel, dim3((unsigned)nchunk), dim3(256), 0, 0,
+                           cnt, nrays, sums);
+        e = hipGetLastError();
+    }
+    std::vector<int64_t> hs(nchunk);
+    if (e == hipSuccess)
+        e = hipMemcpy(hs.data(), sums, nchunk * 8, hipMemcpyDeviceToHost);
+    int64_t run = 0;
+    for (int64_t i = 0; i < nchunk; i++) {
+        const int64_t v = hs[i];
+        hs[i] = run;
+        run += v;
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(sums, hs.data(), nchunk * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(chunk_scan_kernel, dim3((unsigned)nchunk), dim3(256), 0, 0,
+                           cnt, nrays, sums, m->rm_ptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(m->rm_ptr + nrays, &run, 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && run != m->nnz)
+        e = hipErrorUnknown;
+    if (e == hipSuccess) {
+        cursor = reinterpret_cast<unsigned long long *>(cnt);  // reuse as scratch
+        cnt = nullptr;
+        (void)hipFree(cursor);
+        cursor = nullptr;
+        rc = dev_alloc_n(&cursor, nrays, "ray cursors");
+        if (rc) {
+            cleanup();
+            return rc;
+        }
+        e = hipMemcpy(cursor, m->rm_ptr, nrays * 8, hipMemcpyDeviceToDevice);
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(rm_fill_kernel, dim3(gcol), dim3(256), 0, 0, m->col_ptr,
+                           m->idx, m->val, m->val16, m->col_scale, m->ncol, cursor,
+                           m->rm_col, m->rm_val);
+        hipLaunchKernelGGL(rm_sort_kernel, dim3(16384), dim3(256), 0, 0, m->rm_ptr,
+                           nrays, m->rm_col, m->rm_val);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    cleanup();
+    if (e != hipSuccess) {
+        (void)hipFree(m->rm_ptr);
+        (void)hipFree(m->rm_col);
+        (void)hipFree(m->rm_val);
+        m->rm_ptr = nullptr;
+        m->rm_col = nullptr;
+        m->rm_val = nullptr;
+        return fail(CTBB_E_HIP, std::string("ray-major matrix: ") + hipGetErrorString(e));
+    }
+    return CTBB_OK;
+}
+
+// Per data view, the contiguous range of slices whose SSM view window holds
+// the view (the central views are monotone in the slice index).
+void view_slice_ranges(const ctbb_geom *g, const std::vector<int64_t> &si_half,
+                       std::vector<int32_t> *klo, std::vector<int32_t> *khi)
+{
+    klo->assign(g->readings, 0);
+    khi->assign(g->readings, 0);
+    for (int64_t v = 0; v < g->readings; v++) {
+        int32_t lo = -1, hi = -1;
+        for (int64_t k = 0; k < g->nz; k++) {
+            const int64_t i = v - si_half[k];
+            if (i >= 0 && i < g->n_views_sm) {
+                if (lo < 0)
+                    lo = (int32_t)k;
+                hi = (int32_t)k + 1;
+            }
+        }
+        (*klo)[v] = lo < 0 ? 0 : lo;
+        (*khi)[v] = lo < 0 ? 0 : hi;
+    }
+}
+
+template <typename X, int MODE>
+int launch_project(const ctbb_ssm *m, const X *x, const int64_t *d_si_half,
+                   const int32_t *d_klo, const int32_t *d_khi, double *yhat,
+                   const float *y, const float *w, float *e, double *partials,
+                   int grid, hipStream_t st)
+{
+    CTBB_TRY(ensure_ray_major(m));
+    const ctbb_geom *g = &m->g;
+    hipLaunchKernelGGL((project_gather_kernel<X, MODE>), dim3(grid), dim3(kRedBlock), 0,
+                       st, m->rm_ptr, m->rm_col, m->rm_val, x, d_si_half, d_klo, d_khi,
+                       g->n_views_sm, g->n_channels * g->n_rows, g->nx * g->ny,
+                       data_size_of(g), yhat, y, w, e, partials);
     CTBB_HIP(hipGetLastError());
     return CTBB_OK;
 }
@@ -625,13 +973,10 @@ int sum_partials(const double *d_part, int n, hipStream_t st, double *out)
 
 int recon_data_term(ctbb_recon *r, bool write_e, double *data)
 {
-    CTBB_TRY(launch_forward(r->m, r->x, r->slice_off, r->yhat, r->data_size,
-                            r->stream));
     const int g = red_grid(r->data_size);
-    hipLaunchKernelGGL(resync_finish_kernel, dim3(g), dim3(kRedBlock), 0,
-                       r->stream, r->y, r->w, r->yhat, write_e ? r->e : nullptr,
-                       r->data_size, r->partials);
-    CTBB_HIP(hipGetLastError());
+    CTBB_TRY((launch_project<float, 1>(r->m, r->x, r->si_half, r->k_lo, r->k_hi, nullptr,
+                                       r->y, r->w, write_e ? r->e : nullptr,
+                                       r->partials, g, r->stream)));
     return sum_partials(r->partials, g, r->stream, data);
 }
 
@@ -657,30 +1002,41 @@ extern "C" int ctbb_forward_project(const ctbb_ssm *m,
         return fail(CTBB_E_ARG, "null argument");
     const ctbb_geom *g = &m->g;
     const int64_t nvox = g->nx * g->ny * g->nz, ds = data_size_of(g);
-    std::vector<int64_t> off(g->nz);
+    std::vector<int64_t> sih(g->nz);
     for (int64_t k = 0; k < g->nz; k++)
-        off[k] = slice_offset(g, slice_index[k]);
+        sih[k] = slice_offset(g, slice_index[k]) / (g->n_channels * g->n_rows);
+    std::vector<int32_t> klo, khi;
+    view_slice_ranges(g, sih, &klo, &khi);
     double *d_x = nullptr, *d_y = nullptr;
-    int64_t *d_off = nullptr;
+    int64_t *d_sih = nullptr;
+    int32_t *d_klo = nullptr, *d_khi = nullptr;
     int rc = CTBB_OK;
-    if ((rc = dev_alloc_n(&d_x, nvox, "volume")) ||
-        (rc = dev_alloc_n(&d_y, ds, "projection")) ||
-        (rc = dev_alloc_n(&d_off, g->nz, "slice offsets"))) {
+    auto cleanup = [&]() {
         if (d_x) (void)hipFree(d_x);
         if (d_y) (void)hipFree(d_y);
+        if (d_sih) (void)hipFree(d_sih);
+        if (d_klo) (void)hipFree(d_klo);
+        if (d_khi) (void)hipFree(d_khi);
+    };
+    if ((rc = dev_alloc_n(&d_x, nvox, "volume")) ||
+        (rc = dev_alloc_n(&d_y, ds, "projection")) ||
+        (rc = dev_alloc_n(&d_sih, g->nz, "slice view offsets")) ||
+        (rc = dev_alloc_n(&d_klo, g->readings, "view slice ranges")) ||
+        (rc = dev_alloc_n(&d_khi, g->readings, "view slice ranges"))) {
+        cleanup();
         return rc;
     }
     hipError_t e = hipMemcpy(d_x, x, nvox * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(d_off, off.data(), g->nz * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_sih, sih.data(), g->nz * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_klo, klo.data(), g->readings * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_khi, khi.data(), g->readings * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        rc = launch_forward(m, (const double *)d_x, d_off, d_y, ds, 0);
+        rc = launch_project<double, 0>(m, d_x, d_sih, d_klo, d_khi, d_y, nullptr, nullptr,
+                                       nullptr, nullptr, red_grid(ds), 0);
         if (rc == CTBB_OK)
             e = hipMemcpy(yhat, d_y, ds * 8, hipMemcpyDeviceToHost);
     }
-    (void)hipFree(d_x);
-    (void)hipFree(d_y);
-    (void)hipFree(d_off);
+    cleanup();
     if (rc != CTBB_OK)
         return rc;
     CTBB_HIP(e);
@@ -759,14 +1115,121 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
     float *fl[] = {r->y, r->w, r->e, r->x};
     for (float *p : fl)
         if (p) (void)hipFree(p);
-    if (r->yhat) (void)hipFree(r->yhat);
     if (r->slice_off) (void)hipFree(r->slice_off);
     if (r->cols) (void)hipFree(r->cols);
     if (r->batch_ptr) (void)hipFree(r->batch_ptr);
+    if (r->bt_ent) (void)hipFree(r->bt_ent);
+    if (r->bt_val) (void)hipFree(r->bt_val);
+    if (r->bt_vc) (void)hipFree(r->bt_vc);
+    if (r->bt_off) (void)hipFree(r->bt_off);
+    if (r->si_half) (void)hipFree(r->si_half);
+    if (r->k_lo) (void)hipFree(r->k_lo);
+    if (r->k_hi) (void)hipFree(r->k_hi);
     if (r->du) (void)hipFree(r->du);
     if (r->partials) (void)hipFree(r->partials);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
+    return CTBB_OK;
+}
+
+// Transposed copy of every batch's SSM entries for the sample-driven
+// scatter: entries grouped by batch, then by (SSM view, channel) segment,
+// sorted by (row, column slot) inside a segment.
+static int build_transposed_batches(ctbb_recon *r, const std::vector<int32_t> &cols,
+                                    const std::vector<int64_t> &bptr)
+{
+    const ctbb_ssm *m = r->m;
+    const ctbb_geom *g = &r->g;
+    const int64_t nbatch = r->nbatch, ncol = g->nx * g->ny;
+    const int64_t nvc = g->n_views_sm * g->n_channels + 1;
+    if (g->n_rows > 256 || g->n_channels > 1024)
+        return fail(CTBB_E_UNSUPPORTED, "transposed batches need n_rows <= 256 "
+                                        "and n_channels <= 1024");
+    if (r->max_batch > (1 << 14))
+        return fail(CTBB_E_ARG, "batches of more than 16384 columns (tile too "
+                                "small for this volume)");
+    std::vector<int32_t> cb(ncol, -1), cs(ncol, -1);
+    for (int64_t b = 0; b < nbatch; b++)
+        for (int64_t i = bptr[b]; i < bptr[b + 1]; i++) {
+            cb[cols[i]] = (int32_t)b;
+            cs[cols[i]] = (int32_t)(i - bptr[b]);
+        }
+    int32_t *d_cb = nullptr, *d_cs = nullptr;
+    uint32_t *d_cnt = nullptr;
+    int64_t *d_tot = nullptr;
+    int rc = CTBB_OK;
+    auto cleanup = [&]() {
+        if (d_cb) (void)hipFree(d_cb);
+        if (d_cs) (void)hipFree(d_cs);
+        if (d_cnt) (void)hipFree(d_cnt);
+        if (d_tot) (void)hipFree(d_tot);
+    };
+    if ((rc = dev_alloc_n(&d_cb, ncol, "column batches")) ||
+        (rc = dev_alloc_n(&d_cs, ncol, "column slots")) ||
+        (rc = dev_alloc_n(&d_cnt, nbatch * nvc, "segment counts")) ||
+        (rc = dev_alloc_n(&d_tot, nbatch, "batch totals")) ||
+        (rc = dev_alloc_n(&r->bt_vc, nbatch * nvc, "segment index")) ||
+        (rc = dev_alloc_n(&r->bt_off, nbatch, "batch offsets")) ||
+        (rc = dev_alloc_n(&r->bt_ent, std::max<int64_t>(1, r->ssm_fov_entries),
+                          "transposed entries")) ||
+        (rc = dev_alloc_n(&r->bt_val, std::max<int64_t>(1, r->ssm_fov_entries),
+                          "transposed values")) ||
+        (rc = dev_alloc_n(&r->si_half, g->nz, "slice view offsets"))) {
+        cleanup();
+        return rc;
+    }
+    std::vector<int64_t> sih(g->nz);
+    for (int64_t k = 0; k < g->nz; k++)
+        sih[k] = r->slice_off_h[k] / (g->n_channels * g->n_rows);
+    r->h_si_half = sih;
+    std::vector<int32_t> klo, khi;
+    view_slice_ranges(g, sih, &klo, &khi);
+    if ((rc = dev_alloc_n(&r->k_lo, g->readings, "view slice ranges")) ||
+        (rc = dev_alloc_n(&r->k_hi, g->readings, "view slice ranges"))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemcpy(d_cb, cb.data(), ncol * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(r->k_lo, klo.data(), g->readings * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(r->k_hi, khi.data(), g->readings * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_cs, cs.data(), ncol * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(r->si_half, sih.data(), g->nz * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(d_cnt, 0, nbatch * nvc * 4);
+    if (e == hipSuccess) {
+        const int grid = (int)std::min<int64_t>(std::max<int64_t>(1, r->ncols_fov), 65536);
+        hipLaunchKernelGGL(bt_count_kernel, dim3(grid), dim3(256), 0, 0, m->col_ptr,
+                           m->idx, r->cols, r->ncols_fov, d_cb, g->n_rows, nvc, d_cnt);
+        hipLaunchKernelGGL(bt_scan_kernel, dim3((unsigned)nbatch), dim3(1024), 0, 0,
+                           d_cnt, r->bt_vc, d_tot, nvc);
+        e = hipGetLastError();
+    }
+    std::vector<int64_t> tot(nbatch), off(nbatch);
+    if (e == hipSuccess)
+        e = hipMemcpy(tot.data(), d_tot, nbatch * 8, hipMemcpyDeviceToHost);
+    int64_t run = 0;
+    for (int64_t b = 0; b < nbatch; b++) {
+        off[b] = run;
+        run += tot[b];
+    }
+    if (e == hipSuccess && run != r->ssm_fov_entries)
+        e = hipErrorUnknown;
+    if (e == hipSuccess) e = hipMemcpy(r->bt_off, off.data(), nbatch * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(d_cnt, 0, nbatch * nvc * 4);
+    if (e == hipSuccess) {
+        const int grid = (int)std::min<int64_t>(std::max<int64_t>(1, r->ncols_fov), 65536);
+        hipLaunchKernelGGL(bt_fill_kernel, dim3(grid), dim3(256), 0, 0, m->col_ptr,
+                           m->idx, m->val, m->val16, m->col_scale, r->cols,
+                           r->ncols_fov, d_cb, d_cs, g->n_rows, g->n_channels, nvc,
+                           r->bt_vc, r->bt_off, d_cnt, r->bt_ent, r->bt_val);
+        hipLaunchKernelGGL(bt_sort_kernel, dim3(16384), dim3(256), 0, 0, r->bt_vc,
+                           r->bt_off, nbatch, nvc, r->bt_ent, r->bt_val);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    cleanup();
+    if (e != hipSuccess)
+        return fail(CTBB_E_HIP, std::string("transposed batches: ") + hipGetErrorString(e));
     return CTBB_OK;
 }
 
@@ -814,7 +1277,7 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     // apart (never penalty neighbours).  Batches run in raster order of
     // (ly, lx); with tile >= max(nx, ny) every batch is a single column and
     // the sweep is the reference's raster order.
-    const int64_t tile = opts->tile > 0 ? opts->tile : 64;
+    const int64_t tile = opts->tile > 0 ? opts->tile : 8;
     if (tile < 2)
         return bail(fail(CTBB_E_ARG, "tile must be >= 2"));
     const double fov_limit = (g->acquisition_fov / 2.0) * (g->acquisition_fov / 2.0);
@@ -853,7 +1316,6 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     int rc = CTBB_OK;
     if ((rc = dev_alloc_n(&r->y, r->data_size, "sinogram")) ||
         (rc = dev_alloc_n(&r->e, r->data_size, "error sinogram")) ||
-        (rc = dev_alloc_n(&r->yhat, r->data_size, "FP64 projection")) ||
         (rc = dev_alloc_n(&r->x, r->nvox, "volume")) ||
         (rc = dev_alloc_n(&r->slice_off, g->nz, "slice offsets")) ||
         (rc = dev_alloc_n(&r->cols, std::max<int64_t>(1, r->ncols_fov), "FOV columns")) ||
@@ -876,6 +1338,7 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
         CTBB_HIP_R(hipMemcpy(r->w, weights, r->data_size * 4, hipMemcpyHostToDevice));
     CTBB_HIP_R(hipMemset(r->x, 0, r->nvox * 4));
     CTBB_HIP_R(hipMemcpy(r->slice_off, off.data(), g->nz * 8, hipMemcpyHostToDevice));
+    r->slice_off_h = off;
     if (r->ncols_fov)
         CTBB_HIP_R(hipMemcpy(r->cols, cols.data(), r->ncols_fov * 4,
                              hipMemcpyHostToDevice));
@@ -916,6 +1379,8 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     }
     if (r->spacing < 1)
         r->spacing = 1;
+    if ((rc = build_transposed_batches(r, cols, bptr)) != CTBB_OK)
+        return bail(rc);
     *out = r;
     return CTBB_OK;
 }
@@ -1006,7 +1471,17 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                                    : icd_gather_kernel<true, false>)
                        : (weighted ? icd_gather_kernel<false, true>
                                    : icd_gather_kernel<false, false>);
-    auto scatter = fp16 ? icd_scatter_kernel<true> : icd_scatter_kernel<false>;
+    BtArgs bt;
+    bt.ent = r->bt_ent;
+    bt.val = r->bt_val;
+    bt.vc_idx = r->bt_vc;
+    bt.ent_off = r->bt_off;
+    bt.si_half = r->si_half;
+    bt.nsm = r->g.n_views_sm;
+    bt.nch = r->g.n_channels;
+    bt.nr = r->g.n_rows;
+    bt.readings = r->g.readings;
+    const int64_t ncg = (r->g.n_channels + kScatterCh - 1) / kScatterCh;
 
     hipEvent_t e0, e1;
     CTBB_HIP(hipEventCreate(&e0));
@@ -1044,8 +1519,17 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                                    0, r->stream, a, sa);
                 if (samp)
                     CTBB_HIP(hipEventRecord(ev[3 * used + 1], r->stream));
-                hipLaunchKernelGGL(scatter, dim3((unsigned)ntask), dim3(kSweepBlock),
-                                   0, r->stream, a, sa);
+                int64_t vlo = INT64_MAX, vhi = INT64_MIN;
+                for (int64_t k = j; k < r->g.nz; k += r->spacing) {
+                    vlo = std::min(vlo, r->h_si_half[k]);
+                    vhi = std::max(vhi, r->h_si_half[k] + r->g.n_views_sm);
+                }
+                vlo = std::max<int64_t>(vlo, 0);
+                vhi = std::min<int64_t>(vhi, r->g.readings);
+                if (vhi > vlo)
+                    hipLaunchKernelGGL(icd_scatter_kernel, dim3((unsigned)((vhi - vlo) * ncg)),
+                                       dim3(kSweepBlock), 0, r->stream, a, sa, bt, b,
+                                       vlo, ncg);
                 if (samp) {
                     CTBB_HIP(hipEventRecord(ev[3 * used + 2], r->stream));
                     used++;

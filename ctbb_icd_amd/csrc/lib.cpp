@@ -67,3 +67,4 @@ extern "C" int ctbb_init(int device)
     CTBB_HIP(hipSetDevice(device));
     return CTBB_OK;
 }
+

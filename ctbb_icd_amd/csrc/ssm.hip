@@ -623,6 +623,9 @@ extern "C" int ctbb_ssm_free(ctbb_ssm *m)
     if (m->val16) (void)hipFree(m->val16);
     if (m->col_scale) (void)hipFree(m->col_scale);
     if (m->view_off) (void)hipFree(m->view_off);
+    if (m->rm_ptr) (void)hipFree(m->rm_ptr);
+    if (m->rm_col) (void)hipFree(m->rm_col);
+    if (m->rm_val) (void)hipFree(m->rm_val);
     delete m;
     return CTBB_OK;
 }
@@ -643,6 +646,13 @@ extern "C" int ctbb_ssm_to_fp16(ctbb_ssm *m)
     CTBB_HIP(hipFree(m->val));
     m->val = nullptr;
     m->precision = CTBB_SSM_FP16;
+    // the ray-major copy holds the FP32 values: rebuild it from FP16 on use
+    if (m->rm_ptr) (void)hipFree(m->rm_ptr);
+    if (m->rm_col) (void)hipFree(m->rm_col);
+    if (m->rm_val) (void)hipFree(m->rm_val);
+    m->rm_ptr = nullptr;
+    m->rm_col = nullptr;
+    m->rm_val = nullptr;
     return CTBB_OK;
 }
 

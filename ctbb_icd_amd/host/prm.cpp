@@ -250,6 +250,7 @@ int parse_config_checked(const std::string &config_file, recon_params *rp)
     Q(neighbours, size_t);
     Q(resync_every, size_t);
     Q(group_spacing, size_t);
+    Q(tile, size_t);
 #undef Q
     return CTBB_OK;
 }
@@ -282,6 +283,7 @@ static void set_extension_defaults(recon_params *rp)
     rp->neighbours = 8;
     rp->resync_every = 0;
     rp->group_spacing = 0;
+    rp->tile = 0;
 }
 
 int configure_recon_params_checked(const std::string &filename, recon_params *out)

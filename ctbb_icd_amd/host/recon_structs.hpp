@@ -115,6 +115,8 @@ struct recon_params {
     size_t neighbours;             // 8 (reference stencil)
     size_t resync_every;           // FP64 resync period in iterations
     size_t group_spacing;          // 0: derive from the SSM footprints
+    size_t tile;                   // batch tile of the sweep (0: default)
+    std::string prm_path;          // the parameter file itself
 };
 
 struct ct_data {
@@ -126,8 +128,10 @@ struct ct_data {
     double *table_positions;
     double *slice_locations;
     size_t *slice_indices;
-    // extension: statistical weights (null when unweighted)
+    // extension: statistical weights (null when unweighted) and the
+    // device-resident stored system matrix
     float *weights;
+    struct ctbb_ssm *ssm;
 };
 
 // prm handling (src/parse_config.cpp, src/setup.cpp:69-238).  The *_checked

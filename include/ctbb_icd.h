@@ -64,6 +64,11 @@ const char *ctbb_last_error(void);
 int ctbb_init(int device);
 /* Version string of this build. */
 const char *ctbb_version(void);
+/* Measurement helper (no counterpart in the reference): copies `bytes` bytes
+ * between two fresh device buffers with a plain copy kernel
+ * (ctbb_copy_kernel) and reports its duration, so that hardware byte
+ * counters of other kernels can be calibrated against a known traffic. */
+int ctbb_copy_probe(int64_t bytes, double *ms);
 
 /* ---- stored system matrix (SSM) --------------------------------------
  * Replaces generate_system_matrix (src/generate_system_matrix.h:25,
@@ -155,7 +160,7 @@ typedef struct ctbb_icd_opts {
                                  group's footprints are disjoint); across the
                                  batch's columns it is Jacobi (all gather
                                  from the error sinogram as it stood at the
-                                 start of the step).  0 = 64; tile >=
+                                 start of the step).  0 = 8; tile >=
                                  max(nx, ny) gives single-column batches,
                                  i.e. the reference's raster order. */
 } ctbb_icd_opts;

@@ -106,7 +106,7 @@ def forward_project(geom, ssm, slice_index, x):
 
 
 def icd(geom, ssm, raw, weights, slice_index, penalty, lam, delta, spacing,
-        n_iter, x=None, yhat=None, tile=64):
+        n_iter, x=None, yhat=None, tile=8):
     """Run n_iter oracle iterations; returns (x, yhat, updates).
     spacing 0: reference order; spacing >= 2: batched z-group order with the
     given tile (tile >= max(nx, ny): single-column batches)."""

@@ -133,7 +133,7 @@ def test_group_spacing_is_conflict_free(ssm_pair, orc, name):
             assert not orc.column_overlaps(g, host, c, dv), (c, dv)
 
 
-def _icd_case(gpu, ssm_pair, orc, name, n_iter, fp16=False, resync=0, tile=64):
+def _icd_case(gpu, ssm_pair, orc, name, n_iter, fp16=False, resync=0, tile=8):
     s, dev, host = ssm_pair(name)
     g = s.geom
     raw, w = s.synthesize(use_gpu=False)
@@ -154,7 +154,7 @@ def _icd_case(gpu, ssm_pair, orc, name, n_iter, fp16=False, resync=0, tile=64):
 
 @pytest.mark.parametrize("name,n_iter,tile", [
     ("tiny", 3, 1024), ("tiny", 3, 4), ("tiny_ffs", 3, 1024), ("tiny_ffs", 3, 6),
-    ("config0", 1, 64), ("config0", 1, 1024)])
+    ("config0", 1, 16), ("config0", 1, 1024)])
 def test_icd_matches_oracle_same_order(gpu, ssm_pair, orc, name, n_iter, tile):
     """tile 1024: single-column batches (the reference's column order with
     conflict-free slice groups); smaller tiles: multi-column batches."""
