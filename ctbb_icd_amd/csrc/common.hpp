@@ -86,9 +86,10 @@ struct ctbb_ssm {
     int32_t *view_off = nullptr;  // [ncol*n_views_sm] entry offset of each
                                   // view inside its column (or null)
     double build_ms = 0.0;
-    // ray-major copy (CSR over rays), built on first use by the projection
-    int64_t *rm_ptr = nullptr;    // [n_views_sm*n_channels*n_rows + 1]
-    int32_t *rm_col = nullptr;    // [nnz] column of each entry
+    // ray-major copy, built on first use by the projection: entries grouped
+    // by (view, channel), sorted by (row, column) inside a group
+    int64_t *rm_vc = nullptr;     // [n_views_sm*n_channels + 1] group starts
+    uint32_t *rm_ent = nullptr;   // [nnz] column | row << 24
     float *rm_val = nullptr;      // [nnz]
 };
 

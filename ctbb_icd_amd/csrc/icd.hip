@@ -389,16 +389,56 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
     }
 }
 
+// Inclusive sum over runs of equal keys across the lanes of a wavefront:
+// bit l of `head` is set where lane l starts a run (lane 0 always does), and
+// runs are contiguous because keys arrive sorted.  After the call, the last
+// lane of each run holds the sum of its run; the order of additions is fixed
+// (Hillis-Steele), so the result is deterministic.
+template <typename T>
+__device__ __forceinline__ T run_sum(T v, uint64_t head, int lane)
+{
+    const uint64_t upto = head & (~0ull >> (63 - lane));
+    const int start = 63 - __clzll((unsigned long long)upto);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const T u = __shfl_up(v, off, 64);
+        if (lane - off >= start)
+            v += u;
+    }
+    return v;
+}
+
+// Head mask of a wavefront whose lanes hold sorted keys (invalid lanes, at
+// the tail, carry key -1) and run-end test for lane `lane`.
+__device__ __forceinline__ uint64_t run_heads(int key, int lane)
+{
+    const int prev = __shfl_up(key, 1, 64);
+    return __ballot(lane == 0 || prev != key);
+}
+
+__device__ __forceinline__ bool run_end(uint64_t head, int lane)
+{
+    return lane == 63 || ((head >> (lane + 1)) & 1ull);
+}
+
 // Phase B of a step: sample-driven scatter.  Each workgroup owns the
 // contiguous samples of one data view and kScatterCh channels (all rows) and
 // adds up, in LDS, the updates of every voxel of the step that reaches them:
 // for each slice k of the group, the SSM view i = view - (c_k - n_sm/2) and
-// the batch's transposed entries of that view and those channels (sorted by
-// ray, then column slot) give the rays and their column slots.  A ray's
-// entries are summed by one thread; slices are taken one after the other.
-// The workgroup then writes each changed sample once: no two workgroups
-// touch the same sample, so the scatter is conflict-free and deterministic.
+// the batch's transposed entries of that view (sorted by channel, row, then
+// column slot) give the rays and their column slots.  Wavefront w owns the
+// kScatterCh/4 channels [c0 + 8w, c0 + 8w + 8): a slice's entries for them
+// are contiguous (typically under 64), so one 64-lane chunk covers a slice.
+// Lanes form du[slot] * a, runs of equal (channel, row) are summed with
+// shuffles and the last lane of a run adds it into LDS.  The entry loads of
+// kSliceUnroll slices are issued before any of them is reduced, and slices
+// are applied in a fixed order by the one wavefront that owns the channels:
+// no LDS atomics, deterministic sums.  The workgroup then writes each changed
+// sample once; no two workgroups touch the same sample, so the scatter is
+// conflict-free.
 constexpr int kScatterCh = 32;
+constexpr int kWaveCh = kScatterCh / kWaves;
+constexpr int kSliceUnroll = 4;
 constexpr uint32_t kSlotBits = 14, kRowBits = 8;
 
 __device__ __forceinline__ uint32_t bt_slot(uint32_t p) { return p & ((1u << kSlotBits) - 1); }
@@ -414,47 +454,102 @@ struct BtArgs {
     int64_t nsm, nch, nr, readings;
 };
 
+// One 64-entry chunk [p0, hi) of slice v: reduce runs of equal samples and
+// add them into the workgroup tile (acc, rows of nr samples per channel
+// starting at channel c0).
+__device__ __forceinline__ void scatter_chunk(uint32_t e0, float d, bool ok,
+                                              int64_t c0, int nr, int lane,
+                                              float *acc)
+{
+    const int key = ok ? (int)(bt_ch(e0) - c0) * nr + (int)bt_row(e0) : -1;
+    const uint64_t head = run_heads(key, lane);
+    d = run_sum(d, head, lane);
+    if (ok && run_end(head, lane))
+        acc[key] += d;
+}
+
+// LDS: kScatterCh * nr floats (dynamic).
 __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
     SweepArgs a, StepArgs sa, BtArgs bt, int64_t b, int64_t vlo, int64_t ncg)
 {
-    __shared__ float acc[kScatterCh * 256];
-    const int tid = threadIdx.x;
+    extern __shared__ float acc[];
+    __shared__ int any_work;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t view = vlo + blockIdx.x / ncg;
     const int64_t c0 = (blockIdx.x % ncg) * kScatterCh;
     const int64_t c1 = min(c0 + kScatterCh, bt.nch);
-    const int nsamp = (int)((c1 - c0) * bt.nr);
-    for (int i = tid; i < nsamp; i += kSweepBlock)
-        acc[i] = 0.0f;
+    const int64_t wc0 = min(c0 + kWaveCh * wave, c1), wc1 = min(wc0 + kWaveCh, c1);
+    const int nr = (int)bt.nr;
+    const int nsamp = (int)((c1 - c0) * nr);
     const int64_t nvc = bt.nsm * bt.nch + 1;
     const int32_t *vci = bt.vc_idx + b * nvc;
     const uint32_t *ent = bt.ent + bt.ent_off[b];
     const float *val = bt.val + bt.ent_off[b];
-    bool any = false;
-    for (int64_t v = 0; v < sa.ng; v++) {
-        const int64_t k = sa.j + a.spacing * v;
-        const int64_t i = view - bt.si_half[k];
-        if (i < 0 || i >= bt.nsm)
-            continue;
-        const int64_t lo = vci[i * bt.nch + c0], hi = vci[i * bt.nch + c1];
-        if (lo == hi)
-            continue;
-        any = true;
-        __syncthreads();   // previous slice's LDS adds are complete
-        const float *du = a.du + v * sa.nc;
-        for (int64_t p = lo + tid; p < hi; p += kSweepBlock) {
-            const uint32_t e0 = ent[p];
-            const uint32_t ray = e0 >> kSlotBits;
-            if (p > lo && (ent[p - 1] >> kSlotBits) == ray)
-                continue;   // not the first entry of its ray
-            float sum = 0.0f;
-            for (int64_t pp = p; pp < hi && (ent[pp] >> kSlotBits) == ray; pp++)
-                sum += du[bt_slot(ent[pp])] * val[pp];
-            acc[(int)((bt_ch(e0) - c0) * bt.nr + bt_row(e0))] += sum;
+    if (tid == 0)
+        any_work = 0;
+    for (int i = tid; i < nsamp; i += kSweepBlock)
+        acc[i] = 0.0f;
+    __syncthreads();
+    for (int64_t vb = 0; vb < sa.ng; vb += 64) {
+        // lane l: entry range of slice vb + l on this wavefront's channels
+        const int64_t vl = vb + lane;
+        int64_t lo = 0, hi = 0;
+        if (vl < sa.ng && wc1 > wc0) {
+            const int64_t i = view - bt.si_half[sa.j + a.spacing * vl];
+            if (i >= 0 && i < bt.nsm) {
+                lo = vci[i * bt.nch + wc0];
+                hi = vci[i * bt.nch + wc1];
+            }
+        }
+        uint64_t todo = __ballot(hi > lo);
+        if (todo && lane == 0)
+            any_work = 1;
+        while (todo) {
+            int sl[kSliceUnroll];
+            int64_t lo_u[kSliceUnroll], hi_u[kSliceUnroll];
+#pragma unroll
+            for (int u = 0; u < kSliceUnroll; u++) {
+                sl[u] = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
+                todo &= todo ? todo - 1 : 0;
+                lo_u[u] = __shfl(lo, sl[u] < 0 ? 0 : sl[u], 64);
+                hi_u[u] = sl[u] < 0 ? lo_u[u] : __shfl(hi, sl[u], 64);
+            }
+            uint32_t ev[kSliceUnroll];
+            float av[kSliceUnroll];
+            bool ok[kSliceUnroll];
+#pragma unroll
+            for (int u = 0; u < kSliceUnroll; u++) {
+                const int64_t p = lo_u[u] + lane;
+                ok[u] = p < hi_u[u] && lane < 64;
+                ev[u] = ok[u] ? ent[p] : 0u;
+                av[u] = ok[u] ? val[p] : 0.0f;
+            }
+            float dv[kSliceUnroll];
+#pragma unroll
+            for (int u = 0; u < kSliceUnroll; u++) {
+                const float *du = a.du + (vb + (sl[u] < 0 ? 0 : sl[u])) * sa.nc;
+                dv[u] = ok[u] ? du[bt_slot(ev[u])] * av[u] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kSliceUnroll; u++) {
+                if (sl[u] < 0)
+                    continue;
+                scatter_chunk(ev[u], dv[u], ok[u], c0, nr, lane, acc);
+                // slices with more than 64 entries on these channels
+                const float *du = a.du + (vb + sl[u]) * sa.nc;
+                for (int64_t p0 = lo_u[u] + 64; p0 < hi_u[u]; p0 += 64) {
+                    const int64_t p = p0 + lane;
+                    const bool okp = p < hi_u[u];
+                    const uint32_t e0 = okp ? ent[p] : 0u;
+                    const float d = okp ? du[bt_slot(e0)] * val[p] : 0.0f;
+                    scatter_chunk(e0, d, okp, c0, nr, lane, acc);
+                }
+            }
         }
     }
-    if (!any)
-        return;
     __syncthreads();
+    if (!any_work)
+        return;
     float *e = a.e + (view * bt.nch + c0) * bt.nr;
     for (int i = tid; i < nsamp; i += kSweepBlock) {
         const float d = acc[i];
@@ -668,10 +763,14 @@ __global__ void inrange_count_kernel(const int64_t *col_ptr,
 }
 
 // ---------------------------------------------------------------------------
-// Ray-major copy of the SSM (CSR over rays q = row + nr*(ch + nch*view)) and
-// the sample-driven FP64 forward projection built on it: every sample r
-// gathers, for each slice k whose view window contains r's view, the entries
-// of ray q = r - slice offset, in a fixed order (no atomics, deterministic).
+// Ray-major copy of the SSM and the sample-driven FP64 forward projection
+// built on it.  Entries are grouped by (SSM view, channel) and sorted by
+// (row, column) inside a group; an entry packs its column and row into one
+// word, col | row << 24.  Built through per-ray counts (CSR over rays
+// q = row + nr*(ch + nch*view)), of which only the (view, channel) group
+// starts are kept.
+
+constexpr uint32_t kRmColBits = 24, kRmColMask = (1u << kRmColBits) - 1;
 
 __global__ void rm_count_kernel(const int64_t *col_ptr, const int32_t *idx,
                                 int64_t ncol, uint32_t *cnt)
@@ -735,76 +834,112 @@ __global__ __launch_bounds__(256) void chunk_scan_kernel(const uint32_t *cnt,
 
 __global__ void rm_fill_kernel(const int64_t *col_ptr, const int32_t *idx,
                                const float *val, const __half *val16,
-                               const float *col_scale, int64_t ncol,
-                               unsigned long long *cursor, int32_t *rcol,
+                               const float *col_scale, int64_t ncol, int64_t nr,
+                               unsigned long long *cursor, uint32_t *rent,
                                float *rval)
 {
     for (int64_t c = blockIdx.x; c < ncol; c += gridDim.x)
         for (int64_t p = col_ptr[c] + threadIdx.x; p < col_ptr[c + 1]; p += blockDim.x) {
-            const unsigned long long pos = atomicAdd(&cursor[idx[p]], 1ull);
-            rcol[pos] = (int32_t)c;
+            const int32_t q = idx[p];
+            const unsigned long long pos = atomicAdd(&cursor[q], 1ull);
+            rent[pos] = (uint32_t)c | ((uint32_t)(q % nr) << kRmColBits);
             rval[pos] = val16 ? __half2float(val16[p]) * col_scale[c] : val[p];
         }
 }
 
-__global__ void rm_sort_kernel(const int64_t *rptr, int64_t nrays, int32_t *rcol,
+// Insertion sort of every ray by column (the fill order is arbitrary).
+__global__ void rm_sort_kernel(const int64_t *rptr, int64_t nrays, uint32_t *rent,
                                float *rval)
 {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nrays;
          q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t lo = rptr[q], hi = rptr[q + 1];
         for (int64_t i = lo + 1; i < hi; i++) {
-            const int32_t kc = rcol[i];
+            const uint32_t kc = rent[i];
             const float kv = rval[i];
             int64_t j = i - 1;
-            while (j >= lo && rcol[j] > kc) {
-                rcol[j + 1] = rcol[j];
+            while (j >= lo && rent[j] > kc) {
+                rent[j + 1] = rent[j];
                 rval[j + 1] = rval[j];
                 j--;
             }
-            rcol[j + 1] = kc;
+            rent[j + 1] = kc;
             rval[j + 1] = kv;
         }
     }
 }
 
-// Sample-driven FP64 projection.  MODE 0: yhat[r] = (A x)[r].  MODE 1:
-// e[r] = (float)(y[r] - (A x)[r]) (if e) and block partials of
-// w (y - A x)^2.  k_lo/k_hi[view]: slices whose view window holds the view.
+__global__ void rm_group_kernel(const int64_t *rptr, int64_t ngroups, int64_t nr,
+                                int64_t *vc)
+{
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= ngroups;
+         j += (int64_t)gridDim.x * blockDim.x)
+        vc[j] = rptr[j * nr];
+}
+
+// Sample-driven FP64 projection.  One workgroup per (data view, kScatterCh
+// channels), all rows: for each slice k whose view window holds the view,
+// SSM view i = view - (c_k - n_sm/2); wavefront w takes channels c0+w,
+// c0+w+4, ..., reads a channel's entries 64 at a time, forms
+// x[col + nxy k] * a in FP64 per lane, sums runs of equal rows with
+// shuffles and adds each run into an FP64 LDS tile.  Fixed order, no
+// atomics.  MODE 0: yhat = A x.  MODE 1: e = (float)(y - A x) (if e) and
+// one partial of w (y - A x)^2 per workgroup.  LDS: kScatterCh * nr doubles.
 template <typename X, int MODE>
-__global__ __launch_bounds__(kRedBlock) void project_gather_kernel(
-    const int64_t *rptr, const int32_t *rcol, const float *rval, const X *x,
+__global__ __launch_bounds__(kSweepBlock) void project_gather_kernel(
+    const int64_t *vc_ptr, const uint32_t *rent, const float *rval, const X *x,
     const int64_t *si_half, const int32_t *k_lo, const int32_t *k_hi,
-    int64_t nsm, int64_t per_view, int64_t nxy, int64_t data_size,
+    int64_t nsm, int64_t nch, int64_t nr, int64_t nxy, int64_t ncg,
     double *yhat, const float *y, const float *w, float *e, double *partials)
 {
-    __shared__ double sh[kRedBlock / 64];
-    double part = 0.0;
-    for (int64_t r = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; r < data_size;
-         r += (int64_t)gridDim.x * kRedBlock) {
-        const int64_t view = r / per_view, rem = r - view * per_view;
-        double acc = 0.0;
-        for (int32_t k = k_lo[view]; k < k_hi[view]; k++) {
-            const int64_t i = view - si_half[k];
-            if (i < 0 || i >= nsm)
-                continue;
-            const int64_t q = i * per_view + rem;
-            const X *xk = x + nxy * k;
-            for (int64_t p = rptr[q]; p < rptr[q + 1]; p++)
-                acc += (double)xk[rcol[p]] * (double)rval[p];
+    extern __shared__ double pacc[];
+    __shared__ double sh[kWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t view = blockIdx.x / ncg;
+    const int64_t c0 = (blockIdx.x % ncg) * kScatterCh;
+    const int64_t c1 = min(c0 + kScatterCh, nch);
+    const int nsamp = (int)((c1 - c0) * nr);
+    for (int i = tid; i < nsamp; i += kSweepBlock)
+        pacc[i] = 0.0;
+    __syncthreads();
+    for (int32_t k = k_lo[view]; k < k_hi[view]; k++) {
+        const int64_t i = view - si_half[k];
+        if (i < 0 || i >= nsm)
+            continue;
+        const X *xk = x + nxy * k;
+        for (int64_t c = c0 + wave; c < c1; c += kWaves) {
+            const int64_t lo = vc_ptr[i * nch + c], hi = vc_ptr[i * nch + c + 1];
+            double *accc = pacc + (c - c0) * nr;
+            for (int64_t p0 = lo; p0 < hi; p0 += 64) {
+                const int64_t p = p0 + lane;
+                const bool ok = p < hi;
+                const uint32_t e0 = ok ? rent[p] : 0u;
+                const int row = ok ? (int)(e0 >> kRmColBits) : -1;
+                double d = ok ? (double)xk[e0 & kRmColMask] * (double)rval[p] : 0.0;
+                const uint64_t head = run_heads(row, lane);
+                d = run_sum(d, head, lane);
+                if (ok && run_end(head, lane))
+                    accc[row] += d;
+            }
         }
+    }
+    __syncthreads();
+    const int64_t r0 = (view * nch + c0) * nr;
+    double part = 0.0;
+    for (int i = tid; i < nsamp; i += kSweepBlock) {
+        const double acc = pacc[i];
         if (MODE == 0) {
-            yhat[r] = acc;
+            yhat[r0 + i] = acc;
         } else {
-            const double d = (double)y[r] - acc;
+            const double d = (double)y[r0 + i] - acc;
             if (e)
-                e[r] = (float)d;
-            part += (w ? (double)w[r] : 1.0) * d * d;
+                e[r0 + i] = (float)d;
+            part += (w ? (double)w[r0 + i] : 1.0) * d * d;
         }
     }
     if (MODE == 1) {
         const double s = block_sum_f64(part, sh);
-        if (threadIdx.x == 0)
+        if (tid == 0)
             partials[blockIdx.x] = s;
     }
 }
@@ -833,26 +968,41 @@ int red_grid(int64_t n)
 int ensure_ray_major(const ctbb_ssm *mc)
 {
     ctbb_ssm *m = const_cast<ctbb_ssm *>(mc);
-    if (m->rm_ptr)
+    if (m->rm_vc)
         return CTBB_OK;
     const ctbb_geom *g = &m->g;
+    if (g->n_rows > 256 || g->nx * g->ny > (int64_t)kRmColMask + 1)
+        return fail(CTBB_E_UNSUPPORTED, "ray-major matrix needs n_rows <= 256 and "
+                                        "nx*ny <= 2^24");
     const int64_t nrays = g->n_views_sm * g->n_channels * g->n_rows;
+    const int64_t ngroups = g->n_views_sm * g->n_channels;
     uint32_t *cnt = nullptr;
-    int64_t *sums = nullptr;
+    int64_t *sums = nullptr, *rptr = nullptr;
     unsigned long long *cursor = nullptr;
     const int64_t nchunk = (nrays + kScanChunk - 1) / kScanChunk;
     int rc = CTBB_OK;
     auto cleanup = [&]() {
         if (cnt) (void)hipFree(cnt);
         if (sums) (void)hipFree(sums);
+        if (rptr) (void)hipFree(rptr);
         if (cursor) (void)hipFree(cursor);
+    };
+    auto drop = [&]() {
+        if (m->rm_vc) (void)hipFree(m->rm_vc);
+        if (m->rm_ent) (void)hipFree(m->rm_ent);
+        if (m->rm_val) (void)hipFree(m->rm_val);
+        m->rm_vc = nullptr;
+        m->rm_ent = nullptr;
+        m->rm_val = nullptr;
     };
     if ((rc = dev_alloc_n(&cnt, nrays, "ray counts")) ||
         (rc = dev_alloc_n(&sums, nchunk, "chunk sums")) ||
-        (rc = dev_alloc_n(&m->rm_ptr, nrays + 1, "ray pointers")) ||
-        (rc = dev_alloc_n(&m->rm_col, std::max<int64_t>(1, m->nnz), "ray-major columns")) ||
+        (rc = dev_alloc_n(&rptr, nrays + 1, "ray pointers")) ||
+        (rc = dev_alloc_n(&m->rm_vc, ngroups + 1, "ray-major group starts")) ||
+        (rc = dev_alloc_n(&m->rm_ent, std::max<int64_t>(1, m->nnz), "ray-major entries")) ||
         (rc = dev_alloc_n(&m->rm_val, std::max<int64_t>(1, m->nnz), "ray-major values"))) {
         cleanup();
+        drop();
         return rc;
     }
     const int gcol = (int)std::min<int64_t>(m->ncol, 65536);
@@ -877,46 +1027,48 @@ int ensure_ray_major(const ctbb_ssm *mc)
         e = hipMemcpy(sums, hs.data(), nchunk * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(chunk_scan_kernel, dim3((unsigned)nchunk), dim3(256), 0, 0,
-                           cnt, nrays, sums, m->rm_ptr);
+                           cnt, nrays, sums, rptr);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
-        e = hipMemcpy(m->rm_ptr + nrays, &run, 8, hipMemcpyHostToDevice);
+        e = hipMemcpy(rptr + nrays, &run, 8, hipMemcpyHostToDevice);
     if (e == hipSuccess && run != m->nnz)
         e = hipErrorUnknown;
     if (e == hipSuccess) {
-        cursor = reinterpret_cast<unsigned long long *>(cnt);  // reuse as scratch
+        (void)hipFree(cnt);
         cnt = nullptr;
-        (void)hipFree(cursor);
-        cursor = nullptr;
         rc = dev_alloc_n(&cursor, nrays, "ray cursors");
         if (rc) {
             cleanup();
+            drop();
             return rc;
         }
-        e = hipMemcpy(cursor, m->rm_ptr, nrays * 8, hipMemcpyDeviceToDevice);
+        e = hipMemcpy(cursor, rptr, nrays * 8, hipMemcpyDeviceToDevice);
     }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(rm_fill_kernel, dim3(gcol), dim3(256), 0, 0, m->col_ptr,
-                           m->idx, m->val, m->val16, m->col_scale, m->ncol, cursor,
-                           m->rm_col, m->rm_val);
-        hipLaunchKernelGGL(rm_sort_kernel, dim3(16384), dim3(256), 0, 0, m->rm_ptr,
-                           nrays, m->rm_col, m->rm_val);
+                           m->idx, m->val, m->val16, m->col_scale, m->ncol, g->n_rows,
+                           cursor, m->rm_ent, m->rm_val);
+        hipLaunchKernelGGL(rm_sort_kernel, dim3(16384), dim3(256), 0, 0, rptr, nrays,
+                           m->rm_ent, m->rm_val);
+        hipLaunchKernelGGL(rm_group_kernel, dim3(4096), dim3(256), 0, 0, rptr, ngroups,
+                           g->n_rows, m->rm_vc);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
         e = hipDeviceSynchronize();
     cleanup();
     if (e != hipSuccess) {
-        (void)hipFree(m->rm_ptr);
-        (void)hipFree(m->rm_col);
-        (void)hipFree(m->rm_val);
-        m->rm_ptr = nullptr;
-        m->rm_col = nullptr;
-        m->rm_val = nullptr;
+        drop();
         return fail(CTBB_E_HIP, std::string("ray-major matrix: ") + hipGetErrorString(e));
     }
     return CTBB_OK;
+}
+
+// Workgroups of the projection: one per (data view, kScatterCh channels).
+int64_t project_grid(const ctbb_geom *g)
+{
+    return g->readings * ((g->n_channels + kScatterCh - 1) / kScatterCh);
 }
 
 // Per data view, the contiguous range of slices whose SSM view window holds
@@ -945,14 +1097,16 @@ template <typename X, int MODE>
 int launch_project(const ctbb_ssm *m, const X *x, const int64_t *d_si_half,
                    const int32_t *d_klo, const int32_t *d_khi, double *yhat,
                    const float *y, const float *w, float *e, double *partials,
-                   int grid, hipStream_t st)
+                   hipStream_t st)
 {
     CTBB_TRY(ensure_ray_major(m));
     const ctbb_geom *g = &m->g;
-    hipLaunchKernelGGL((project_gather_kernel<X, MODE>), dim3(grid), dim3(kRedBlock), 0,
-                       st, m->rm_ptr, m->rm_col, m->rm_val, x, d_si_half, d_klo, d_khi,
-                       g->n_views_sm, g->n_channels * g->n_rows, g->nx * g->ny,
-                       data_size_of(g), yhat, y, w, e, partials);
+    const int64_t ncg = (g->n_channels + kScatterCh - 1) / kScatterCh;
+    const size_t lds = sizeof(double) * kScatterCh * g->n_rows;
+    hipLaunchKernelGGL((project_gather_kernel<X, MODE>), dim3((unsigned)project_grid(g)),
+                       dim3(kSweepBlock), lds, st, m->rm_vc, m->rm_ent, m->rm_val, x,
+                       d_si_half, d_klo, d_khi, g->n_views_sm, g->n_channels, g->n_rows,
+                       g->nx * g->ny, ncg, yhat, y, w, e, partials);
     CTBB_HIP(hipGetLastError());
     return CTBB_OK;
 }
@@ -973,10 +1127,10 @@ int sum_partials(const double *d_part, int n, hipStream_t st, double *out)
 
 int recon_data_term(ctbb_recon *r, bool write_e, double *data)
 {
-    const int g = red_grid(r->data_size);
+    const int g = (int)project_grid(&r->g);
     CTBB_TRY((launch_project<float, 1>(r->m, r->x, r->si_half, r->k_lo, r->k_hi, nullptr,
                                        r->y, r->w, write_e ? r->e : nullptr,
-                                       r->partials, g, r->stream)));
+                                       r->partials, r->stream)));
     return sum_partials(r->partials, g, r->stream, data);
 }
 
@@ -1032,7 +1186,7 @@ extern "C" int ctbb_forward_project(const ctbb_ssm *m,
     if (e == hipSuccess) e = hipMemcpy(d_khi, khi.data(), g->readings * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         rc = launch_project<double, 0>(m, d_x, d_sih, d_klo, d_khi, d_y, nullptr, nullptr,
-                                       nullptr, nullptr, red_grid(ds), 0);
+                                       nullptr, nullptr, 0);
         if (rc == CTBB_OK)
             e = hipMemcpy(yhat, d_y, ds * 8, hipMemcpyDeviceToHost);
     }
@@ -1319,7 +1473,8 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
         (rc = dev_alloc_n(&r->x, r->nvox, "volume")) ||
         (rc = dev_alloc_n(&r->slice_off, g->nz, "slice offsets")) ||
         (rc = dev_alloc_n(&r->cols, std::max<int64_t>(1, r->ncols_fov), "FOV columns")) ||
-        (rc = dev_alloc_n(&r->partials, 4096, "reduction partials")) ||
+        (rc = dev_alloc_n(&r->partials, std::max<int64_t>(4096, project_grid(&r->g)),
+                           "reduction partials")) ||
         (rc = dev_alloc_n(&r->batch_ptr, bptr.size(), "batch pointers")) ||
         (rc = dev_alloc_n(&r->du, (max_batch + 8) * g->nz, "step updates")))
         return bail(rc);
@@ -1482,6 +1637,7 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
     bt.nr = r->g.n_rows;
     bt.readings = r->g.readings;
     const int64_t ncg = (r->g.n_channels + kScatterCh - 1) / kScatterCh;
+    const size_t scatter_lds = sizeof(float) * kScatterCh * r->g.n_rows;
 
     hipEvent_t e0, e1;
     CTBB_HIP(hipEventCreate(&e0));
@@ -1528,7 +1684,7 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                 vhi = std::min<int64_t>(vhi, r->g.readings);
                 if (vhi > vlo)
                     hipLaunchKernelGGL(icd_scatter_kernel, dim3((unsigned)((vhi - vlo) * ncg)),
-                                       dim3(kSweepBlock), 0, r->stream, a, sa, bt, b,
+                                       dim3(kSweepBlock), scatter_lds, r->stream, a, sa, bt, b,
                                        vlo, ncg);
                 if (samp) {
                     CTBB_HIP(hipEventRecord(ev[3 * used + 2], r->stream));

@@ -623,8 +623,8 @@ extern "C" int ctbb_ssm_free(ctbb_ssm *m)
     if (m->val16) (void)hipFree(m->val16);
     if (m->col_scale) (void)hipFree(m->col_scale);
     if (m->view_off) (void)hipFree(m->view_off);
-    if (m->rm_ptr) (void)hipFree(m->rm_ptr);
-    if (m->rm_col) (void)hipFree(m->rm_col);
+    if (m->rm_vc) (void)hipFree(m->rm_vc);
+    if (m->rm_ent) (void)hipFree(m->rm_ent);
     if (m->rm_val) (void)hipFree(m->rm_val);
     delete m;
     return CTBB_OK;
@@ -647,11 +647,11 @@ extern "C" int ctbb_ssm_to_fp16(ctbb_ssm *m)
     m->val = nullptr;
     m->precision = CTBB_SSM_FP16;
     // the ray-major copy holds the FP32 values: rebuild it from FP16 on use
-    if (m->rm_ptr) (void)hipFree(m->rm_ptr);
-    if (m->rm_col) (void)hipFree(m->rm_col);
+    if (m->rm_vc) (void)hipFree(m->rm_vc);
+    if (m->rm_ent) (void)hipFree(m->rm_ent);
     if (m->rm_val) (void)hipFree(m->rm_val);
-    m->rm_ptr = nullptr;
-    m->rm_col = nullptr;
+    m->rm_vc = nullptr;
+    m->rm_ent = nullptr;
     m->rm_val = nullptr;
     return CTBB_OK;
 }
