@@ -130,14 +130,16 @@ def main():
     wall = time.perf_counter() - t
     weighted = w is not None
     entry_bytes = 6 if args.fp16 else 8
-    steps = st.sweep_launches // 2                 # gather + scatter per step
+    steps = st.sweep_launches // 3                 # gather, solve, scatter per step
     visits = st.entry_visits / steps               # in-range entries per step
-    # algorithmic bytes per launch: gather reads e (+ w) at every in-range
-    # entry plus each FOV column's SSM entries once per sweep; scatter writes e
+    # algorithmic bytes per launch (per in-range entry: 4 B error read [+ 4 B
+    # weight read] in the gather, 4 B error write in the scatter; each FOV
+    # column's SSM entries once per sweep, charged to the gather)
     gather_bytes = visits * (8 if weighted else 4) + st.ssm_entries * entry_bytes * args.steps / steps
     scatter_bytes = visits * 4
     kernels = {
         "icd_gather_kernel": (gather_bytes, st.gather_ms_avg),
+        "icd_solve_kernel": (0.0, st.solve_ms_avg),
         "icd_scatter_kernel": (scatter_bytes, st.scatter_ms_avg),
     }
     dom = max(kernels, key=lambda k: kernels[k][1])

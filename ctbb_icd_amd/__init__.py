@@ -76,7 +76,7 @@ class IcdStats(C.Structure):
         ("iterations", _i64), ("voxel_updates", _i64), ("entry_visits", _i64),
         ("sweep_ms", _f64), ("resync_ms", _f64), ("cost", _f64),
         ("sweep_launches", _i64), ("ssm_entries", _i64),
-        ("gather_ms_avg", _f64), ("scatter_ms_avg", _f64),
+        ("gather_ms_avg", _f64), ("scatter_ms_avg", _f64), ("solve_ms_avg", _f64),
     ]
 
 

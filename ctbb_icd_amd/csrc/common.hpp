@@ -107,7 +107,22 @@ struct ctbb_recon {
     int64_t *slice_off = nullptr;  // [nz] sample offsets
     int32_t *cols = nullptr;    // FOV columns, grouped by batch
     int64_t *batch_ptr = nullptr;  // [nbatch+1] offsets into cols
+    float *ew = nullptr;        // weighted error w e (weighted recons)
     float *du = nullptr;        // per-step voxel updates
+    double *th2 = nullptr;      // theta2 per voxel [nvox], iteration-invariant
+    bool th2_valid = false;     // th2 filled (by the first sweep)
+    // view-blocked batch copy of the matrix (gather): entries of batch b
+    // ordered by (view block ib, column slot), view blocks of kViewBlock views
+    int32_t *gb_idx = nullptr;    // [fov entries] ray index (as in the SSM)
+    float *gb_val = nullptr;      // [fov entries] FP32 value (FP32 matrices)
+    __half *gb_val16 = nullptr;   // [fov entries] FP16 value (FP16 matrices)
+    uint16_t *gb_slot = nullptr;  // [fov entries] column slot in the batch
+    int64_t *gb_ptr = nullptr;    // [nib*ncols_fov+1] start of (b, ib, slot)
+    int64_t nib = 0;              // view blocks
+    uint32_t *gb_pairs = nullptr; // per group j: (view block | member << 16)
+    std::vector<int64_t> h_pair_off;    //   in gather dispatch order; [spacing+1]
+    float *part1 = nullptr;       // per-step partials [ng][nib][max_batch]
+    float *part2 = nullptr;       //   (theta1, theta2)
     int64_t ncols_fov = 0;
     int64_t nbatch = 0;
     int64_t max_batch = 0;

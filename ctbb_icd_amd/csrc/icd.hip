@@ -114,17 +114,21 @@ __device__ double edge_preserving_step(int64_t q, double alpha, double beta,
     }
     const double tol = 0.00000001;
     double c = (a + b) / 2.0;
+    // f(a) is carried instead of re-evaluated: a only ever takes the value
+    // of c, whose f was just computed with the same arguments
+    double f_a = ep_func(a, alpha, beta, delta, lambda, diff, used);
     for (int it = 0; it < 4096; it++) {
         c = (a + b) / 2.0;
         const double f_c = ep_func(c, alpha, beta, delta, lambda, diff, used);
-        const double f_a = ep_func(a, alpha, beta, delta, lambda, diff, used);
         if ((f_c == 0) || (b - a) / 2 < tol)
             return c;
         const int sc = f_c < 0 ? -1 : 1, sa = f_a < 0 ? -1 : 1;
-        if (sc == sa)
+        if (sc == sa) {
             a = c;
-        else
+            f_a = f_c;
+        } else {
             b = c;
+        }
     }
     return c;
 }
@@ -174,14 +178,14 @@ struct SweepArgs {
     const float *val;
     const __half *val16;
     const float *col_scale;
-    float *e;
-    const float *w;
+    float *e;                 // error sinogram y - A x
+    float *ew;                // weighted error w (y - A x) (== e unweighted)
+    const float *w;           // weights or null
     float *x;
-    float *du;                // [max tasks per step]
+    float *du;                // [ng*nc] updates of the step
+    double *th2;              // [nvox] theta2 per voxel (cached)
     const int64_t *slice_off;
     const int32_t *cols;      // batch columns, concatenated
-    const int64_t *batch_ptr; // [nbatch+1]
-    int64_t step_begin, step_end;  // step = batch * spacing + j
     int64_t spacing;
     int64_t nz, nx, nxy, nvox, data_size;
     int penalty;
@@ -274,121 +278,6 @@ __global__ void penalty_steps_wave_kernel(int penalty, double lambda,
         du[i] = d;
 }
 
-constexpr int kUnroll = 8;
-constexpr int kXcd = 8;   // workgroups are dealt round-robin to the 8 XCDs
-
-// Task t of a step -> (column slot ci, group member v).  With at least 8
-// columns in the batch, the ng tasks of one column go to workgroups of the
-// same XCD (t = 8 s + xcd), so the column's SSM entries are fetched into
-// one L2; smaller batches use t = ci * ng + v.
-__device__ __forceinline__ bool task_of(int64_t t, int64_t ng, int64_t nc,
-                                        int64_t &ci, int64_t &v)
-{
-    if (nc < kXcd) {
-        ci = t / ng;
-        v = t - ci * ng;
-        return ci < nc;
-    }
-    const int64_t xcd = t % kXcd, sl = t / kXcd;
-    ci = (sl / ng) * kXcd + xcd;
-    v = sl - (sl / ng) * ng;
-    return ci < nc;
-}
-
-struct StepArgs {
-    int64_t c0, nc, j, ng;
-};
-
-// Phase A of a step: one workgroup per voxel task gathers the theta sums
-// from the error sinogram (unchanged during the phase), solves the penalty
-// step in FP64 (wavefront 0), clamps and writes x and du.
-template <bool FP16, bool WEIGHTED>
-__global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
-                                                                 StepArgs sa)
-{
-    __shared__ float red1[kWaves], red2[kWaves];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int64_t t = blockIdx.x;
-    int64_t ci, v;
-    if (!task_of(t, sa.ng, sa.nc, ci, v))
-        return;
-    const int64_t col = a.cols[sa.c0 + ci];
-    const int64_t k = sa.j + a.spacing * v;
-    const int64_t off = a.slice_off[k];
-    const int64_t beg = a.col_ptr[col], len = a.col_ptr[col + 1] - beg;
-    const float scale = FP16 ? a.col_scale[col] : 1.0f;
-    const int32_t *__restrict__ idx = a.idx + beg;
-    const float *__restrict__ e = a.e;
-    const float *__restrict__ w = a.w;
-    const int64_t ds = a.data_size;
-    float t1 = 0.0f, t2 = 0.0f;
-    // branch-free: every lane issues kUnroll index/value loads, then kUnroll
-    // error (and weight) loads; out-of-range entries read e[0] with a zero
-    // coefficient
-    for (int64_t base = tid; base < len; base += kSweepBlock * kUnroll) {
-        int32_t iv[kUnroll];
-        float av[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const int64_t o = base + u * kSweepBlock;
-            const int64_t oc = o < len ? o : len - 1;
-            iv[u] = idx[oc];
-            av[u] = ssm_value<FP16>(a, beg + oc, scale);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        int64_t r[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const int64_t o = base + u * kSweepBlock;
-            r[u] = (int64_t)iv[u] + off;
-            const bool ok = o < len && r[u] >= 0 && r[u] < ds;
-            av[u] = ok ? av[u] : 0.0f;
-            r[u] = ok ? r[u] : 0;
-        }
-        float ev[kUnroll], wv[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            ev[u] = e[r[u]];
-            wv[u] = WEIGHTED ? w[r[u]] : 1.0f;
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const float wa = WEIGHTED ? wv[u] * av[u] : av[u];
-            t2 += wa * av[u];
-            t1 += wa * ev[u];
-        }
-    }
-    t1 = wave_sum(t1);
-    t2 = wave_sum(t2);
-    if (lane == 0) {
-        red1[tid >> 6] = t1;
-        red2[tid >> 6] = t2;
-    }
-    __syncthreads();
-    if (tid < 64) {
-        double th1 = 0.0, th2 = 0.0;
-        for (int i = 0; i < kWaves; i++) {
-            th1 += (double)red1[i];
-            th2 += (double)red2[i];
-        }
-        const int64_t q = col + a.nxy * k;
-        double du;
-        if (a.penalty == CTBB_PENALTY_QUADRATIC)
-            du = quadratic_step(q, th2, th1, a.lambda, a.x, a.nx, a.nvox);
-        else
-            du = edge_preserving_wave(q, th2, th1, a.lambda, a.delta, a.x, a.nx,
-                                      a.nvox, lane);
-        if (tid == 0) {
-            const double xq = (double)a.x[q];
-            if (du + xq < 0)
-                du = -xq;
-            a.x[q] = (float)(xq + du);
-            a.du[v * sa.nc + ci] = (float)du;
-        }
-    }
-}
-
 // Inclusive sum over runs of equal keys across the lanes of a wavefront:
 // bit l of `head` is set where lane l starts a run (lane 0 always does), and
 // runs are contiguous because keys arrive sorted.  After the call, the last
@@ -419,6 +308,223 @@ __device__ __forceinline__ uint64_t run_heads(int key, int lane)
 __device__ __forceinline__ bool run_end(uint64_t head, int lane)
 {
     return lane == 63 || ((head >> (lane + 1)) & 1ull);
+}
+
+constexpr int kXcd = 8;   // workgroups are dealt round-robin to the 8 XCDs
+
+struct StepArgs {
+    int64_t c0, nc, j, ng;
+};
+
+// ---------------------------------------------------------------------------
+// Phase A1 of a step: view-blocked gather.
+//
+// The SSM views are cut into blocks of kViewBlock views; for every batch the
+// matrix entries of its columns are copied in (view block ib, column slot)
+// order, so the entries of one block and a run of slots are contiguous.  A
+// workgroup owns one block and kSweepBlock slots (wavefront w: slots
+// [64w, 64w + 64) of the chunk).  It reads the block's entries once per
+// slice of the group -- from L2 after the first slice -- and for slice k
+// sums a * (w e) at sample idx + slice offset; lanes hold consecutive
+// entries, runs of equal slots are summed with shuffles and the last lane
+// of a run adds into the slot's LDS accumulator (one wavefront per slot, in
+// entry order: deterministic).  The per-(slice, block, slot) sums go to a
+// dense partials array; phase A2 adds them over the blocks.  The
+// workgroups of one step read the error sinogram in bands of kViewBlock
+// views per slice, and the blocks are dispatched in residue classes of M
+// (ib, ib + M, ib + 2M, ...), M ~ view distance of consecutive slices of the
+// group / kViewBlock, so the bands of consecutive workgroups overlap for all
+// but one slice; logical workgroups go to the XCDs in contiguous ranges.
+// FP16 matrices: values are summed unscaled and the per-column scale is
+// applied in phase A2 (it factors out of both sums).
+constexpr int kViewBlock = 8;
+constexpr int kGatherUnroll = 4;
+
+struct GatherArgs {
+    const int32_t *idx;       // view-blocked copy (see above)
+    const float *val;
+    const __half *val16;
+    const uint16_t *slot;
+    const int64_t *ptr;       // global (b, ib, slot) starts
+    int64_t base;             // nib * (first FOV index of the batch)
+    float *part1, *part2;     // [ng][nib][nc]
+    int64_t nib;
+    const int64_t *si_half;   // [nz] central view - n_sm/2
+    const uint32_t *pairs;    // (ib | member << 16) of the step's group, in
+    int64_t npairs;           //   dispatch order
+    int64_t nsc;              // slot chunks
+    int64_t nsm, readings;
+};
+
+// View block ib of slice half-offset h holds data views [8 ib + h,
+// min(8 ib + 8, n_sm) + h); true if any of them is inside [0, readings).
+__host__ __device__ inline bool block_valid(int64_t ib, int64_t h, int64_t nsm,
+                                            int64_t readings)
+{
+    const int64_t v0 = ib * kViewBlock;
+    const int64_t v1 = v0 + kViewBlock < nsm ? v0 + kViewBlock : nsm;
+    return v0 + h < readings && v1 + h > 0;
+}
+
+template <bool FP16>
+__device__ __forceinline__ float gb_value(const GatherArgs &g, int64_t p)
+{
+    return FP16 ? __half2float(g.val16[p]) : g.val[p];
+}
+
+template <bool FP16, bool THETA2>
+__global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
+                                                                 StepArgs sa,
+                                                                 GatherArgs g)
+{
+    __shared__ float acc1[kSweepBlock], acc2[kSweepBlock];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t nwg = g.npairs * g.nsc;
+    const int64_t per_xcd = (nwg + kXcd - 1) / kXcd;
+    const int64_t L = (int64_t)(blockIdx.x % kXcd) * per_xcd + blockIdx.x / kXcd;
+    if (L >= nwg)
+        return;
+    const int64_t pi = L / g.nsc, sc = L - pi * g.nsc;
+    const uint32_t pr = g.pairs[pi];
+    const int64_t ib = pr & 0xffffu, v = pr >> 16;
+    const int64_t sw0 = sc * kSweepBlock + wave * 64;
+    if (sw0 >= sa.nc)
+        return;
+    const int64_t sw1 = min(sw0 + 64, sa.nc);
+    const int64_t *ptr = g.ptr + g.base + ib * sa.nc;
+    const int64_t lo = ptr[sw0], hi = ptr[sw1];
+    float *a1 = acc1 + wave * 64, *a2 = acc2 + wave * 64;
+    const float *__restrict__ ew = a.ew;
+    const float *__restrict__ w = a.w;
+    const int64_t ds = a.data_size;
+    const int64_t k = sa.j + a.spacing * v;
+    const int64_t off = a.slice_off[k];
+    a1[lane] = 0.0f;
+    if (THETA2)
+        a2[lane] = 0.0f;
+    for (int64_t p0 = lo; p0 < hi; p0 += 64 * kGatherUnroll) {
+        int32_t iv[kGatherUnroll], sl[kGatherUnroll];
+        float av[kGatherUnroll];
+        bool ok[kGatherUnroll];
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; u++) {
+            const int64_t p = p0 + u * 64 + lane;
+            ok[u] = p < hi;
+            iv[u] = ok[u] ? g.idx[p] : 0;
+            av[u] = ok[u] ? gb_value<FP16>(g, p) : 0.0f;
+            sl[u] = ok[u] ? (int32_t)g.slot[p] : -1;
+        }
+        int64_t r[kGatherUnroll];
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; u++) {
+            r[u] = (int64_t)iv[u] + off;
+            const bool in = ok[u] && r[u] >= 0 && r[u] < ds;
+            av[u] = in ? av[u] : 0.0f;
+            r[u] = in ? r[u] : 0;
+        }
+        float ev[kGatherUnroll], wv[kGatherUnroll];
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; u++) {
+            ev[u] = ew[r[u]];
+            wv[u] = (THETA2 && w) ? w[r[u]] : 1.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; u++) {
+            if (p0 + u * 64 >= hi)
+                break;
+            const uint64_t head = run_heads(sl[u], lane);
+            const float t1 = run_sum(av[u] * ev[u], head, lane);
+            const float t2 = THETA2 ? run_sum((wv[u] * av[u]) * av[u], head, lane) : 0.0f;
+            if (ok[u] && run_end(head, lane)) {
+                a1[sl[u] - sw0] += t1;
+                if (THETA2)
+                    a2[sl[u] - sw0] += t2;
+            }
+        }
+    }
+    if (sw0 + lane < sw1) {
+        const int64_t o = (v * g.nib + ib) * sa.nc + sw0 + lane;
+        g.part1[o] = a1[lane];
+        if (THETA2)
+            g.part2[o] = a2[lane];
+    }
+}
+
+// Phase A2 of a step: per voxel, theta1 (and, on the first sweep, theta2)
+// is the FP64 sum of its partials over the view blocks (wavefront w takes
+// blocks w, w+4, ...; the four sums are added in wavefront order).  Then one
+// thread per voxel takes the penalty step with the reference's solver in
+// FP64, clamps to x >= 0 and writes x and the update du.  A workgroup covers
+// 64 slots of one group member.
+template <bool FP16, bool THETA2>
+__global__ __launch_bounds__(kSweepBlock) void icd_solve_kernel(SweepArgs a,
+                                                                StepArgs sa,
+                                                                GatherArgs g)
+{
+    __shared__ double s1[kWaves][64], s2[kWaves][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t nch64 = (sa.nc + 63) / 64;
+    const int64_t v = blockIdx.x / nch64;
+    const int64_t s = (blockIdx.x - v * nch64) * 64 + lane;
+    double t1 = 0.0, t2 = 0.0;
+    const int64_t k = sa.j + a.spacing * v;
+    // valid view blocks of this slice: a contiguous range [ib0, ib1)
+    const int64_t h = g.si_half[k];
+    int64_t ib0 = 0, ib1 = g.nib;
+    while (ib0 < ib1 && !block_valid(ib0, h, g.nsm, g.readings))
+        ib0++;
+    while (ib1 > ib0 && !block_valid(ib1 - 1, h, g.nsm, g.readings))
+        ib1--;
+    if (s < sa.nc) {
+        const int64_t o = v * g.nib * sa.nc + s;
+        int64_t ib = ib0 + wave;
+        for (; ib + 3 * kWaves < ib1; ib += 4 * kWaves) {
+            float q1[4], q2[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                q1[u] = g.part1[o + (ib + u * kWaves) * sa.nc];
+                q2[u] = THETA2 ? g.part2[o + (ib + u * kWaves) * sa.nc] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                t1 += (double)q1[u];
+                t2 += (double)q2[u];
+            }
+        }
+        for (; ib < ib1; ib += kWaves) {
+            t1 += (double)g.part1[o + ib * sa.nc];
+            if (THETA2)
+                t2 += (double)g.part2[o + ib * sa.nc];
+        }
+    }
+    s1[wave][lane] = t1;
+    s2[wave][lane] = t2;
+    __syncthreads();
+    if (wave != 0 || s >= sa.nc)
+        return;
+    double th1 = 0.0, th2 = 0.0;
+    for (int i = 0; i < kWaves; i++) {
+        th1 += s1[i][lane];
+        th2 += s2[i][lane];
+    }
+    const int64_t col = a.cols[sa.c0 + s];
+    const int64_t q = col + a.nxy * k;
+    if (FP16) {
+        const double sc = (double)a.col_scale[col];
+        th1 *= sc;
+        th2 *= sc * sc;
+    }
+    if (THETA2)
+        a.th2[q] = th2;
+    else
+        th2 = a.th2[q];
+    double du = penalty_step(a.penalty, q, th2, th1, a.lambda, a.delta, a.x, a.nx,
+                             a.nvox);
+    const double xq = (double)a.x[q];
+    if (du + xq < 0)
+        du = -xq;
+    a.x[q] = (float)(xq + du);
+    a.du[v * sa.nc + s] = (float)du;
 }
 
 // Phase B of a step: sample-driven scatter.  Each workgroup owns the
@@ -550,11 +656,24 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
     __syncthreads();
     if (!any_work)
         return;
-    float *e = a.e + (view * bt.nch + c0) * bt.nr;
-    for (int i = tid; i < nsamp; i += kSweepBlock) {
-        const float d = acc[i];
-        if (d != 0.0f)
-            e[i] = e[i] - d;
+    const int64_t r0 = (view * bt.nch + c0) * bt.nr;
+    float *e = a.e + r0;
+    if (a.w) {
+        float *ew = a.ew + r0;
+        const float *w = a.w + r0;
+        for (int i = tid; i < nsamp; i += kSweepBlock) {
+            const float d = acc[i];
+            if (d != 0.0f) {
+                e[i] = e[i] - d;
+                ew[i] = ew[i] - w[i] * d;
+            }
+        }
+    } else {
+        for (int i = tid; i < nsamp; i += kSweepBlock) {
+            const float d = acc[i];
+            if (d != 0.0f)
+                e[i] = e[i] - d;
+        }
     }
 }
 
@@ -681,24 +800,13 @@ __device__ double block_sum_f64(double v, double *sh)
     return s;
 }
 
-// e = y - yhat (optional) and per-block partial sums of w (y - yhat)^2.
-__global__ __launch_bounds__(kRedBlock) void resync_finish_kernel(
-    const float *y, const float *w, const double *yhat, float *e,
-    int64_t data_size, double *partials)
+// ew = w e (FP32 of the FP64 product).
+__global__ __launch_bounds__(kRedBlock) void weighted_error_kernel(
+    const float *e, const float *w, int64_t n, float *ew)
 {
-    __shared__ double sh[kRedBlock / 64];
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x;
-         i < data_size; i += (int64_t)gridDim.x * kRedBlock) {
-        const double d = (double)y[i] - yhat[i];
-        if (e)
-            e[i] = (float)d;
-        const double ww = w ? (double)w[i] : 1.0;
-        acc += ww * d * d;
-    }
-    const double s = block_sum_f64(acc, sh);
-    if (threadIdx.x == 0)
-        partials[blockIdx.x] = s;
+    for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kRedBlock)
+        ew[i] = (float)((double)w[i] * (double)e[i]);
 }
 
 // Penalty part of the cost: sum over pairs (q, q+d), d in {1, nx-1, nx, nx+1}.
@@ -890,7 +998,8 @@ __global__ __launch_bounds__(kSweepBlock) void project_gather_kernel(
     const int64_t *vc_ptr, const uint32_t *rent, const float *rval, const X *x,
     const int64_t *si_half, const int32_t *k_lo, const int32_t *k_hi,
     int64_t nsm, int64_t nch, int64_t nr, int64_t nxy, int64_t ncg,
-    double *yhat, const float *y, const float *w, float *e, double *partials)
+    double *yhat, const float *y, const float *w, float *e, float *ew,
+    double *partials)
 {
     extern __shared__ double pacc[];
     __shared__ double sh[kWaves];
@@ -932,9 +1041,12 @@ __global__ __launch_bounds__(kSweepBlock) void project_gather_kernel(
             yhat[r0 + i] = acc;
         } else {
             const double d = (double)y[r0 + i] - acc;
+            const double wi = w ? (double)w[r0 + i] : 1.0;
             if (e)
                 e[r0 + i] = (float)d;
-            part += (w ? (double)w[r0 + i] : 1.0) * d * d;
+            if (ew)
+                ew[r0 + i] = (float)(wi * d);
+            part += wi * d * d;
         }
     }
     if (MODE == 1) {
@@ -1096,7 +1208,8 @@ void view_slice_ranges(const ctbb_geom *g, const std::vector<int64_t> &si_half,
 template <typename X, int MODE>
 int launch_project(const ctbb_ssm *m, const X *x, const int64_t *d_si_half,
                    const int32_t *d_klo, const int32_t *d_khi, double *yhat,
-                   const float *y, const float *w, float *e, double *partials,
+                   const float *y, const float *w, float *e, float *ew,
+                   double *partials,
                    hipStream_t st)
 {
     CTBB_TRY(ensure_ray_major(m));
@@ -1106,7 +1219,7 @@ int launch_project(const ctbb_ssm *m, const X *x, const int64_t *d_si_half,
     hipLaunchKernelGGL((project_gather_kernel<X, MODE>), dim3((unsigned)project_grid(g)),
                        dim3(kSweepBlock), lds, st, m->rm_vc, m->rm_ent, m->rm_val, x,
                        d_si_half, d_klo, d_khi, g->n_views_sm, g->n_channels, g->n_rows,
-                       g->nx * g->ny, ncg, yhat, y, w, e, partials);
+                       g->nx * g->ny, ncg, yhat, y, w, e, ew, partials);
     CTBB_HIP(hipGetLastError());
     return CTBB_OK;
 }
@@ -1130,6 +1243,7 @@ int recon_data_term(ctbb_recon *r, bool write_e, double *data)
     const int g = (int)project_grid(&r->g);
     CTBB_TRY((launch_project<float, 1>(r->m, r->x, r->si_half, r->k_lo, r->k_hi, nullptr,
                                        r->y, r->w, write_e ? r->e : nullptr,
+                                       write_e ? r->ew : nullptr,
                                        r->partials, r->stream)));
     return sum_partials(r->partials, g, r->stream, data);
 }
@@ -1186,7 +1300,7 @@ extern "C" int ctbb_forward_project(const ctbb_ssm *m,
     if (e == hipSuccess) e = hipMemcpy(d_khi, khi.data(), g->readings * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         rc = launch_project<double, 0>(m, d_x, d_sih, d_klo, d_khi, d_y, nullptr, nullptr,
-                                       nullptr, nullptr, 0);
+                                       nullptr, nullptr, nullptr, 0);
         if (rc == CTBB_OK)
             e = hipMemcpy(yhat, d_y, ds * 8, hipMemcpyDeviceToHost);
     }
@@ -1281,6 +1395,16 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
     if (r->k_hi) (void)hipFree(r->k_hi);
     if (r->du) (void)hipFree(r->du);
     if (r->partials) (void)hipFree(r->partials);
+    if (r->ew) (void)hipFree(r->ew);
+    if (r->th2) (void)hipFree(r->th2);
+    if (r->part1) (void)hipFree(r->part1);
+    if (r->part2) (void)hipFree(r->part2);
+    if (r->gb_idx) (void)hipFree(r->gb_idx);
+    if (r->gb_val) (void)hipFree(r->gb_val);
+    if (r->gb_val16) (void)hipFree(r->gb_val16);
+    if (r->gb_slot) (void)hipFree(r->gb_slot);
+    if (r->gb_ptr) (void)hipFree(r->gb_ptr);
+    if (r->gb_pairs) (void)hipFree(r->gb_pairs);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return CTBB_OK;
@@ -1289,6 +1413,146 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
 // Transposed copy of every batch's SSM entries for the sample-driven
 // scatter: entries grouped by batch, then by (SSM view, channel) segment,
 // sorted by (row, column slot) inside a segment.
+// View-blocked batch copy (gather): entry counts and copy of every (FOV
+// column, view block); fov_batch/fov_slot/fov_base give the batch, slot and
+// nib * (first FOV index of the batch) of each FOV column index.
+__global__ void gb_count_kernel(const int64_t *col_ptr, const int32_t *view_off,
+                                const int32_t *cols, int64_t ncols,
+                                const int64_t *fov_base, const int32_t *fov_nc,
+                                const int32_t *fov_slot, int64_t nsm, int64_t nib,
+                                uint32_t *cnt)
+{
+    const int64_t n = ncols * nib;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = t / nib, ib = t - i * nib;
+        const int64_t col = cols[i];
+        const int32_t *vo = view_off + col * nsm;
+        const int64_t v0 = ib * kViewBlock, v1 = min(v0 + kViewBlock, nsm);
+        const int64_t e0 = vo[v0];
+        const int64_t e1 = v1 < nsm ? (int64_t)vo[v1] : col_ptr[col + 1] - col_ptr[col];
+        cnt[fov_base[i] + ib * fov_nc[i] + fov_slot[i]] = (uint32_t)(e1 - e0);
+    }
+}
+
+__global__ void gb_fill_kernel(const int64_t *col_ptr, const int32_t *idx,
+                               const float *val, const __half *val16,
+                               const int32_t *view_off, const int32_t *cols,
+                               int64_t ncols, const int64_t *fov_base,
+                               const int32_t *fov_nc, const int32_t *fov_slot,
+                               int64_t nsm, int64_t nib, const int64_t *gptr,
+                               int32_t *gidx, float *gval, __half *gval16,
+                               uint16_t *gslot)
+{
+    const int64_t n = ncols * nib;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = t / nib, ib = t - i * nib;
+        const int64_t col = cols[i];
+        const int64_t src = col_ptr[col] + view_off[col * nsm + ib * kViewBlock];
+        const int64_t o = fov_base[i] + ib * fov_nc[i] + fov_slot[i];
+        const int64_t dst = gptr[o], len = gptr[o + 1] - dst;
+        for (int64_t q = 0; q < len; q++) {
+            gidx[dst + q] = idx[src + q];
+            if (gval16)
+                gval16[dst + q] = val16[src + q];
+            else
+                gval[dst + q] = val[src + q];
+            gslot[dst + q] = (uint16_t)fov_slot[i];
+        }
+    }
+}
+
+static int build_gather_blocks(ctbb_recon *r, const std::vector<int64_t> &bptr)
+{
+    const ctbb_ssm *m = r->m;
+    const ctbb_geom *g = &r->g;
+    const int64_t nsm = g->n_views_sm, nib = (nsm + kViewBlock - 1) / kViewBlock;
+    const int64_t ncols = r->ncols_fov, n = nib * ncols;
+    const bool fp16 = m->precision == CTBB_SSM_FP16;
+    r->nib = nib;
+    std::vector<int64_t> base(ncols);
+    std::vector<int32_t> nc(ncols), slot(ncols);
+    for (int64_t b = 0; b < r->nbatch; b++)
+        for (int64_t i = bptr[b]; i < bptr[b + 1]; i++) {
+            base[i] = nib * bptr[b];
+            nc[i] = (int32_t)(bptr[b + 1] - bptr[b]);
+            slot[i] = (int32_t)(i - bptr[b]);
+        }
+    int64_t *d_base = nullptr, *sums = nullptr;
+    int32_t *d_nc = nullptr, *d_slot = nullptr;
+    uint32_t *cnt = nullptr;
+    const int64_t nchunk = (n + kScanChunk - 1) / kScanChunk;
+    int rc = CTBB_OK;
+    auto cleanup = [&]() {
+        if (d_base) (void)hipFree(d_base);
+        if (sums) (void)hipFree(sums);
+        if (d_nc) (void)hipFree(d_nc);
+        if (d_slot) (void)hipFree(d_slot);
+        if (cnt) (void)hipFree(cnt);
+    };
+    const int64_t ne = std::max<int64_t>(1, r->ssm_fov_entries);
+    if ((rc = dev_alloc_n(&d_base, std::max<int64_t>(1, ncols), "FOV batch bases")) ||
+        (rc = dev_alloc_n(&d_nc, std::max<int64_t>(1, ncols), "FOV batch sizes")) ||
+        (rc = dev_alloc_n(&d_slot, std::max<int64_t>(1, ncols), "FOV slots")) ||
+        (rc = dev_alloc_n(&cnt, std::max<int64_t>(1, n), "block counts")) ||
+        (rc = dev_alloc_n(&sums, std::max<int64_t>(1, nchunk), "chunk sums")) ||
+        (rc = dev_alloc_n(&r->gb_ptr, n + 1, "block pointers")) ||
+        (rc = dev_alloc_n(&r->gb_idx, ne, "blocked indices")) ||
+        (rc = dev_alloc_n(&r->gb_slot, ne, "blocked slots")) ||
+        (rc = fp16 ? dev_alloc_n(&r->gb_val16, ne, "blocked values")
+                   : dev_alloc_n(&r->gb_val, ne, "blocked values"))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (ncols) {
+        e = hipMemcpy(d_base, base.data(), ncols * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_nc, nc.data(), ncols * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_slot, slot.data(), ncols * 4, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && n) {
+        hipLaunchKernelGGL(gb_count_kernel, dim3(4096), dim3(256), 0, 0, m->col_ptr,
+                           m->view_off, r->cols, ncols, d_base, d_nc, d_slot, nsm, nib, cnt);
+        hipLaunchKernelGGL(chunk_sum_kernel, dim3((unsigned)nchunk), dim3(256), 0, 0, cnt,
+                           n, sums);
+        e = hipGetLastError();
+    }
+    std::vector<int64_t> hs(nchunk);
+    if (e == hipSuccess && nchunk)
+        e = hipMemcpy(hs.data(), sums, nchunk * 8, hipMemcpyDeviceToHost);
+    int64_t run = 0;
+    for (int64_t i = 0; i < nchunk; i++) {
+        const int64_t v = hs[i];
+        hs[i] = run;
+        run += v;
+    }
+    if (e == hipSuccess && nchunk)
+        e = hipMemcpy(sums, hs.data(), nchunk * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && n) {
+        hipLaunchKernelGGL(chunk_scan_kernel, dim3((unsigned)nchunk), dim3(256), 0, 0, cnt,
+                           n, sums, r->gb_ptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(r->gb_ptr + n, &run, 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && run != r->ssm_fov_entries)
+        e = hipErrorUnknown;
+    if (e == hipSuccess && n) {
+        hipLaunchKernelGGL(gb_fill_kernel, dim3(8192), dim3(256), 0, 0, m->col_ptr, m->idx,
+                           m->val, m->val16, m->view_off, r->cols, ncols, d_base, d_nc,
+                           d_slot, nsm, nib, r->gb_ptr, r->gb_idx, r->gb_val, r->gb_val16,
+                           r->gb_slot);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    cleanup();
+    if (e != hipSuccess)
+        return fail(CTBB_E_HIP, std::string("view-blocked batches: ") + hipGetErrorString(e));
+    return CTBB_OK;
+}
+
 static int build_transposed_batches(ctbb_recon *r, const std::vector<int32_t> &cols,
                                     const std::vector<int64_t> &bptr)
 {
@@ -1478,7 +1742,8 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
         (rc = dev_alloc_n(&r->batch_ptr, bptr.size(), "batch pointers")) ||
         (rc = dev_alloc_n(&r->du, (max_batch + 8) * g->nz, "step updates")))
         return bail(rc);
-    if (weights && (rc = dev_alloc_n(&r->w, r->data_size, "weights")))
+    if (weights && ((rc = dev_alloc_n(&r->w, r->data_size, "weights")) ||
+                    (rc = dev_alloc_n(&r->ew, r->data_size, "weighted error"))))
         return bail(rc);
 #define CTBB_HIP_R(call)                                                      \
     do {                                                                      \
@@ -1489,8 +1754,12 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     } while (0)
     CTBB_HIP_R(hipMemcpy(r->y, raw, r->data_size * 4, hipMemcpyHostToDevice));
     CTBB_HIP_R(hipMemcpy(r->e, raw, r->data_size * 4, hipMemcpyHostToDevice));
-    if (weights)
+    if (weights) {
         CTBB_HIP_R(hipMemcpy(r->w, weights, r->data_size * 4, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(weighted_error_kernel, dim3(red_grid(r->data_size)),
+                           dim3(kRedBlock), 0, 0, r->e, r->w, r->data_size, r->ew);
+        CTBB_HIP_R(hipGetLastError());
+    }
     CTBB_HIP_R(hipMemset(r->x, 0, r->nvox * 4));
     CTBB_HIP_R(hipMemcpy(r->slice_off, off.data(), g->nz * 8, hipMemcpyHostToDevice));
     r->slice_off_h = off;
@@ -1536,6 +1805,49 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
         r->spacing = 1;
     if ((rc = build_transposed_batches(r, cols, bptr)) != CTBB_OK)
         return bail(rc);
+    if ((rc = build_gather_blocks(r, bptr)) != CTBB_OK)
+        return bail(rc);
+    // partials and theta2 cache; per slice group, the gather's (view block,
+    // member) pairs ordered by the first data view they touch, so that
+    // consecutive workgroups read the same band of the error sinogram
+    {
+        const int64_t ng_max = (g->nz + r->spacing - 1) / r->spacing;
+        const int64_t np = std::max<int64_t>(1, ng_max * r->nib * max_batch);
+        if ((rc = dev_alloc_n(&r->part1, np, "partial sums")) ||
+            (rc = dev_alloc_n(&r->part2, np, "partial sums")) ||
+            (rc = dev_alloc_n(&r->th2, r->nvox, "theta2")))
+            return bail(rc);
+        r->th2_valid = false;
+        if (r->nib > 0xffff || ng_max > 0xffff)
+            return bail(fail(CTBB_E_UNSUPPORTED, "too many view blocks or slices per group"));
+        std::vector<uint32_t> pairs;
+        r->h_pair_off.assign(1, 0);
+        for (int64_t j = 0; j < r->spacing; j++) {
+            std::vector<std::pair<int64_t, uint32_t>> pj;
+            for (int64_t k = j, v = 0; k < g->nz; k += r->spacing, v++)
+                for (int64_t ib = 0; ib < r->nib; ib++)
+                    if (block_valid(ib, r->h_si_half[k], g->n_views_sm, g->readings))
+                        pj.emplace_back(ib * kViewBlock + r->h_si_half[k],
+                                        (uint32_t)ib | ((uint32_t)v << 16));
+            std::stable_sort(pj.begin(), pj.end(),
+                             [](const std::pair<int64_t, uint32_t> &x,
+                                const std::pair<int64_t, uint32_t> &y) {
+                                 return x.first < y.first;
+                             });
+            for (auto &q : pj)
+                pairs.push_back(q.second);
+            r->h_pair_off.push_back((int64_t)pairs.size());
+        }
+        if ((rc = dev_alloc_n(&r->gb_pairs, std::max<size_t>(1, pairs.size()), "gather pairs")))
+            return bail(rc);
+        if (!pairs.empty()) {
+            hipError_t he2 = hipMemcpy(r->gb_pairs, pairs.data(), pairs.size() * 4,
+                                       hipMemcpyHostToDevice);
+            if (he2 != hipSuccess)
+                return bail(fail(CTBB_E_HIP, std::string("gather pairs: ") +
+                                                 hipGetErrorString(he2)));
+        }
+    }
     *out = r;
     return CTBB_OK;
 }
@@ -1619,13 +1931,21 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
     a.delta = r->opts.delta;
 
     a.du = r->du;
-    a.batch_ptr = r->batch_ptr;
-    const bool fp16 = m->precision == CTBB_SSM_FP16;
-    const bool weighted = r->w != nullptr;
-    auto gather = fp16 ? (weighted ? icd_gather_kernel<true, true>
-                                   : icd_gather_kernel<true, false>)
-                       : (weighted ? icd_gather_kernel<false, true>
-                                   : icd_gather_kernel<false, false>);
+    a.ew = r->w ? r->ew : r->e;
+    a.th2 = r->th2;
+    const bool fp16 = r->gb_val16 != nullptr;
+    GatherArgs ga;
+    ga.idx = r->gb_idx;
+    ga.val = r->gb_val;
+    ga.val16 = r->gb_val16;
+    ga.slot = r->gb_slot;
+    ga.ptr = r->gb_ptr;
+    ga.part1 = r->part1;
+    ga.part2 = r->part2;
+    ga.nib = r->nib;
+    ga.si_half = r->si_half;
+    ga.nsm = r->g.n_views_sm;
+    ga.readings = r->g.readings;
     BtArgs bt;
     bt.ent = r->bt_ent;
     bt.val = r->bt_val;
@@ -1644,17 +1964,27 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
     CTBB_HIP(hipEventCreate(&e1));
     double sweep_ms = 0.0, resync_ms = 0.0;
     int64_t launches = 0;
-    // Per-kernel durations: events around the gather and the scatter launch
-    // of every `every`-th step (at most kSamples steps per iteration).
+    // Per-kernel durations: events around the three launches of every
+    // `every`-th step (at most kSamples steps per iteration).
     constexpr int kSamples = 128;
     const int64_t steps_per_it = r->nbatch * std::min<int64_t>(r->spacing, r->g.nz);
     const int64_t every = std::max<int64_t>(1, steps_per_it / kSamples);
-    std::vector<hipEvent_t> ev(3 * kSamples);
+    std::vector<hipEvent_t> ev(4 * kSamples);
     for (auto &x : ev)
         CTBB_HIP(hipEventCreate(&x));
-    double g_ms = 0.0, s_ms = 0.0;
+    double g_ms = 0.0, v_ms = 0.0, s_ms = 0.0;
     int64_t n_samp = 0;
     for (int it = 0; it < n; it++) {
+        // the first sweep also computes and caches theta2
+        const bool first = !r->th2_valid;
+        auto gather = fp16 ? (first ? icd_gather_kernel<true, true>
+                                    : icd_gather_kernel<true, false>)
+                           : (first ? icd_gather_kernel<false, true>
+                                    : icd_gather_kernel<false, false>);
+        auto solve = fp16 ? (first ? icd_solve_kernel<true, true>
+                                   : icd_solve_kernel<true, false>)
+                          : (first ? icd_solve_kernel<false, true>
+                                   : icd_solve_kernel<false, false>);
         CTBB_HIP(hipEventRecord(e0, r->stream));
         int64_t step = 0, used = 0;
         for (int64_t b = 0; b < r->nbatch; b++) {
@@ -1665,16 +1995,12 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                 sa.nc = nc;
                 sa.j = j;
                 sa.ng = (r->g.nz - j + r->spacing - 1) / r->spacing;
-                const int64_t ntask = nc < kXcd
-                                          ? nc * sa.ng
-                                          : ((nc + kXcd - 1) / kXcd) * kXcd * sa.ng;
-                const bool samp = (step % every) == 0 && used < kSamples;
-                if (samp)
-                    CTBB_HIP(hipEventRecord(ev[3 * used], r->stream));
-                hipLaunchKernelGGL(gather, dim3((unsigned)ntask), dim3(kSweepBlock),
-                                   0, r->stream, a, sa);
-                if (samp)
-                    CTBB_HIP(hipEventRecord(ev[3 * used + 1], r->stream));
+                ga.base = r->nib * c0;
+                ga.pairs = r->gb_pairs + r->h_pair_off[j];
+                ga.npairs = r->h_pair_off[j + 1] - r->h_pair_off[j];
+                ga.nsc = (nc + kSweepBlock - 1) / kSweepBlock;
+                const int64_t nwg = ga.npairs * ga.nsc;
+                const int64_t ntask = kXcd * ((nwg + kXcd - 1) / kXcd);
                 int64_t vlo = INT64_MAX, vhi = INT64_MIN;
                 for (int64_t k = j; k < r->g.nz; k += r->spacing) {
                     vlo = std::min(vlo, r->h_si_half[k]);
@@ -1682,18 +2008,31 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                 }
                 vlo = std::max<int64_t>(vlo, 0);
                 vhi = std::min<int64_t>(vhi, r->g.readings);
+                const bool samp = (step % every) == 0 && used < kSamples;
+                if (samp)
+                    CTBB_HIP(hipEventRecord(ev[4 * used], r->stream));
+                if (ntask > 0)
+                    hipLaunchKernelGGL(gather, dim3((unsigned)ntask), dim3(kSweepBlock), 0,
+                                       r->stream, a, sa, ga);
+                if (samp)
+                    CTBB_HIP(hipEventRecord(ev[4 * used + 1], r->stream));
+                hipLaunchKernelGGL(solve, dim3((unsigned)(sa.ng * ((nc + 63) / 64))),
+                                   dim3(kSweepBlock), 0, r->stream, a, sa, ga);
+                if (samp)
+                    CTBB_HIP(hipEventRecord(ev[4 * used + 2], r->stream));
                 if (vhi > vlo)
                     hipLaunchKernelGGL(icd_scatter_kernel, dim3((unsigned)((vhi - vlo) * ncg)),
                                        dim3(kSweepBlock), scatter_lds, r->stream, a, sa, bt, b,
                                        vlo, ncg);
                 if (samp) {
-                    CTBB_HIP(hipEventRecord(ev[3 * used + 2], r->stream));
+                    CTBB_HIP(hipEventRecord(ev[4 * used + 3], r->stream));
                     used++;
                 }
-                launches += 2;
+                launches += 3;
                 step++;
             }
         }
+        r->th2_valid = true;
         CTBB_HIP(hipGetLastError());
         CTBB_HIP(hipEventRecord(e1, r->stream));
         CTBB_HIP(hipEventSynchronize(e1));
@@ -1701,10 +2040,12 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
         CTBB_HIP(hipEventElapsedTime(&ms, e0, e1));
         sweep_ms += ms;
         for (int64_t i = 0; i < used; i++) {
-            float ga = 0.0f, sc = 0.0f;
-            CTBB_HIP(hipEventElapsedTime(&ga, ev[3 * i], ev[3 * i + 1]));
-            CTBB_HIP(hipEventElapsedTime(&sc, ev[3 * i + 1], ev[3 * i + 2]));
+            float ga = 0.0f, so = 0.0f, sc = 0.0f;
+            CTBB_HIP(hipEventElapsedTime(&ga, ev[4 * i], ev[4 * i + 1]));
+            CTBB_HIP(hipEventElapsedTime(&so, ev[4 * i + 1], ev[4 * i + 2]));
+            CTBB_HIP(hipEventElapsedTime(&sc, ev[4 * i + 2], ev[4 * i + 3]));
             g_ms += ga;
+            v_ms += so;
             s_ms += sc;
         }
         n_samp += used;
@@ -1724,6 +2065,7 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
         (void)hipEventDestroy(x);
     if (stats) {
         stats->gather_ms_avg = n_samp ? g_ms / n_samp : 0.0;
+        stats->solve_ms_avg = n_samp ? v_ms / n_samp : 0.0;
         stats->scatter_ms_avg = n_samp ? s_ms / n_samp : 0.0;
         stats->iterations = n;
         stats->voxel_updates = (int64_t)n * r->ncols_fov * r->g.nz;

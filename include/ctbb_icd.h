@@ -172,13 +172,14 @@ typedef struct ctbb_icd_stats {
     double sweep_ms;          /* device time of the ICD sweep kernels     */
     double resync_ms;         /* device time of FP64 resyncs              */
     double cost;              /* cost after the last iteration (FP64)     */
-    int64_t sweep_launches;   /* kernel launches of the sweeps (a gather
-                                 and a scatter launch per step)          */
+    int64_t sweep_launches;   /* kernel launches of the sweeps (partial
+                                 gather, solve and scatter per step)     */
     int64_t ssm_entries;      /* SSM entries of the FOV columns (each read
                                  once per column per sweep, algorithmically) */
-    double gather_ms_avg;     /* average gather-launch duration (HIP events
-                                 around sampled steps)                   */
+    double gather_ms_avg;     /* average partial-gather launch duration
+                                 (HIP events around sampled steps)       */
     double scatter_ms_avg;    /* average scatter-launch duration          */
+    double solve_ms_avg;      /* average solve-launch duration            */
 } ctbb_icd_stats;
 
 /* raw: measured line integrals y [readings*n_channels*n_rows], reference
