@@ -8,12 +8,13 @@ device entry point fails loudly when no HIP device is present.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "libctbb_icd.so"
+LIB_PATH = Path(os.environ.get("CTBB_ICD_LIBRARY", str(PKG / "libctbb_icd.so")))
 
 CTBB_SSM_FP32 = 0
 CTBB_SSM_FP16 = 1

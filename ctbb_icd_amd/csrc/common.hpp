@@ -121,6 +121,7 @@ struct ctbb_recon {
     int64_t nib = 0;              // view blocks
     uint32_t *gb_pairs = nullptr; // per group j: (view block | member << 16)
     std::vector<int64_t> h_pair_off;    //   in gather dispatch order; [spacing+1]
+    std::vector<int64_t> view_stride;   // [spacing] scatter dispatch stride
     float *part1 = nullptr;       // per-step partials [ng][nib][max_batch]
     float *part2 = nullptr;       //   (theta1, theta2)
     int64_t ncols_fov = 0;

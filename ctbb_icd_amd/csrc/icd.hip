@@ -1,30 +1,33 @@
 // icd.hip -- iterative coordinate descent (ICD) on gfx950.
 //
-// Replaces icd_iteration (src/icd_iteration.cpp:42-286) with the penalty
-// steps of src/penalties.cpp:39-154 fused into the sweep kernel.
+// Replaces icd_iteration (src/icd_iteration.cpp:42-286), with the penalty
+// steps of src/penalties.cpp:39-154 evaluated per voxel in FP64.
 //
 // State (device resident, reference sinogram layout
 // r = row + nr*(channel + nch*view)):
-//   e  FP32 error sinogram y - A x (the reference keeps yhat = A x in FP64
-//      and forms raw - yhat on the fly, src/icd_iteration.cpp:209)
-//   x  FP32 volume in the rotated frame, index c + nx*ny*k
-//   w  optional FP32 statistical weights (w == 1 reproduces the reference)
+//   e   FP32 error sinogram y - A x (the reference keeps yhat = A x in FP64
+//       and forms raw - yhat on the fly, src/icd_iteration.cpp:209)
+//   ew  FP32 weighted error w e (weighted reconstructions; e itself when
+//       unweighted), the only sinogram the gather reads
+//   x   FP32 volume in the rotated frame, index c + nx*ny*k
+//   w   optional FP32 statistical weights (w == 1 reproduces the reference)
 //
-// Sweep order: columns c = ix + nx*iy in raster order inside the acquisition
-// FOV (src/icd_iteration.cpp:157-181); in each column the slices are visited
-// as groups {j, j+s, j+2s, ...}, j = 0..s-1, where the spacing s is chosen so
-// that the sinogram footprints of every group are pairwise disjoint (and no
-// two members are penalty neighbours).  One group is one step of a
-// persistent cooperative kernel: one workgroup per voxel of the group
-// gathers theta1 = sum w a e and theta2 = sum w a^2 over the column's
-// in-range entries (wave shuffles + LDS, fixed order), takes the penalty
-// step in FP64, clamps to x >= 0, writes x and scatters e -= du a.  The
-// scatter is conflict-free because the group's footprints are disjoint; a
-// grid-wide barrier separates the steps.  The result equals the sequential
-// sweep in the group order (an oracle replays it), up to FP32 rounding.
+// Sweep order: the FOV columns (src/icd_iteration.cpp:157-181) are cut into
+// batches -- the column at one local position of every tile x tile block of
+// the image -- run in raster order of that position; in each batch the
+// slices are visited as groups {j, j+s, j+2s, ...}, j = 0..s-1, where the
+// spacing s keeps the sinogram footprints of one column's group members
+// apart (and no two members are penalty neighbours).  A step is one (batch,
+// group): phase A1 gathers theta1 for every voxel of the step from the error
+// sinogram as it stood at the start of the step, phase A2 solves the
+// penalty step per voxel and writes x and du, phase B scatters
+// e -= du a, sample-driven and conflict-free.  Across the batch's columns the
+// step is Jacobi, along a column Gauss-Seidel in group order; with tile >=
+// max(nx, ny) batches are single columns and the order is the reference's.
+// The oracle replays exactly this order.
 //
-// FP64 is used for the per-voxel scalar penalty solve, the forward
-// projection of the periodic resync e = y - A x, and the cost.
+// FP64 is used for the per-voxel sums of partials and the penalty solve, the
+// forward projection of the periodic resync e = y - A x, and the cost.
 #include "common.hpp"
 
 #include <hip/hip_fp16.h>
@@ -158,16 +161,7 @@ __global__ void penalty_steps_kernel(int penalty, double lambda, double delta,
 }
 
 // ---------------------------------------------------------------------------
-// Sweep kernel.
-//
-// One step = (batch b, slice group j): the voxels (c, k) for every column c
-// of batch b and every slice k = j, j+s, j+2s, ... .  Phase A: one workgroup
-// per voxel gathers theta1 = sum w a e and theta2 = sum w a^2 over the
-// column's in-range entries from the error sinogram as it stood at the start
-// of the step, solves the penalty step, clamps and writes x and du.  Phase B:
-// every voxel scatters e -= du a (atomic adds: columns of one batch may
-// share rays at the crossings of their sinogram traces; slices of one column
-// never do).  Grid-wide barriers separate the phases and the steps.
+// Sweep kernels (gather, solve, scatter): see the phase comments below.
 
 constexpr int kSweepBlock = 256;
 constexpr int kWaves = kSweepBlock / 64;
@@ -322,29 +316,34 @@ struct StepArgs {
 // The SSM views are cut into blocks of kViewBlock views; for every batch the
 // matrix entries of its columns are copied in (view block ib, column slot)
 // order, so the entries of one block and a run of slots are contiguous.  A
-// workgroup owns one block and kSweepBlock slots (wavefront w: slots
-// [64w, 64w + 64) of the chunk).  It reads the block's entries once per
-// slice of the group -- from L2 after the first slice -- and for slice k
-// sums a * (w e) at sample idx + slice offset; lanes hold consecutive
-// entries, runs of equal slots are summed with shuffles and the last lane
-// of a run adds into the slot's LDS accumulator (one wavefront per slot, in
-// entry order: deterministic).  The per-(slice, block, slot) sums go to a
-// dense partials array; phase A2 adds them over the blocks.  The
-// workgroups of one step read the error sinogram in bands of kViewBlock
-// views per slice, and the blocks are dispatched in residue classes of M
-// (ib, ib + M, ib + 2M, ...), M ~ view distance of consecutive slices of the
-// group / kViewBlock, so the bands of consecutive workgroups overlap for all
-// but one slice; logical workgroups go to the XCDs in contiguous ranges.
-// FP16 matrices: values are summed unscaled and the per-column scale is
-// applied in phase A2 (it factors out of both sums).
-constexpr int kViewBlock = 8;
-constexpr int kGatherUnroll = 4;
+// workgroup takes one (view block, group member) pair and kSweepBlock slots
+// (wavefront w: slots [64w, 64w + 64) of the chunk).  Lanes hold consecutive
+// entries and form a * (w e) at sample idx + slice offset (and, on the first
+// sweep, w a^2); runs of equal slots are summed with shuffles and the last
+// lane of a run adds into the slot's LDS accumulator (one wavefront per
+// slot, in entry order: deterministic).  The per-(member, block, slot) sums
+// go to a dense partials array; phase A2 adds them over the blocks in a
+// fixed order.  The pairs are dispatched in order of the first data view
+// they touch (ib * kViewBlock + c_k - n_sm/2), so workgroups resident at the
+// same time read the same band of a few data views of the error sinogram --
+// a band that fits in L2 -- whichever slice they work on; logical workgroups
+// go to the XCDs in contiguous ranges.  FP16 matrices: values are summed
+// unscaled and the per-column scale is applied in phase A2 (it factors out
+// of both sums).
+#ifndef CTBB_VIEW_BLOCK
+#define CTBB_VIEW_BLOCK 4
+#endif
+#ifndef CTBB_GATHER_UNROLL
+#define CTBB_GATHER_UNROLL 4
+#endif
+constexpr int kViewBlock = CTBB_VIEW_BLOCK;
+constexpr int kGatherUnroll = CTBB_GATHER_UNROLL;
 
 struct GatherArgs {
     const int32_t *idx;       // view-blocked copy (see above)
     const float *val;
     const __half *val16;
-    const uint16_t *slot;
+    const uint16_t *slot;     // column slot of each entry
     const int64_t *ptr;       // global (b, ib, slot) starts
     int64_t base;             // nib * (first FOV index of the batch)
     float *part1, *part2;     // [ng][nib][nc]
@@ -528,23 +527,35 @@ __global__ __launch_bounds__(kSweepBlock) void icd_solve_kernel(SweepArgs a,
 }
 
 // Phase B of a step: sample-driven scatter.  Each workgroup owns the
-// contiguous samples of one data view and kScatterCh channels (all rows) and
-// adds up, in LDS, the updates of every voxel of the step that reaches them:
-// for each slice k of the group, the SSM view i = view - (c_k - n_sm/2) and
-// the batch's transposed entries of that view (sorted by channel, row, then
-// column slot) give the rays and their column slots.  Wavefront w owns the
-// kScatterCh/4 channels [c0 + 8w, c0 + 8w + 8): a slice's entries for them
-// are contiguous (typically under 64), so one 64-lane chunk covers a slice.
+// contiguous samples of kScatterViews consecutive data views and kScatterCh
+// channels (all rows) and adds up, in LDS, the updates of every voxel of the
+// step that reaches them: for a data view and a slice k of the group, the
+// SSM view i = view - (c_k - n_sm/2) and the batch's transposed entries of
+// that view (sorted by channel, row, then column slot) give the rays and
+// their column slots.  Wavefront w owns the channels [c0 + 8w, c0 + 8w + 8)
+// of every view of the workgroup: one (view, slice) item's entries for them
+// are contiguous (typically under 64), so one 64-lane chunk covers an item.
 // Lanes form du[slot] * a, runs of equal (channel, row) are summed with
 // shuffles and the last lane of a run adds it into LDS.  The entry loads of
-// kSliceUnroll slices are issued before any of them is reduced, and slices
-// are applied in a fixed order by the one wavefront that owns the channels:
-// no LDS atomics, deterministic sums.  The workgroup then writes each changed
+// kItemUnroll items are issued before any of them is reduced, and items are
+// applied in a fixed order by the one wavefront that owns the channels: no
+// LDS atomics, deterministic sums.  The workgroup then writes each changed
 // sample once; no two workgroups touch the same sample, so the scatter is
 // conflict-free.
-constexpr int kScatterCh = 32;
+// Tile and unroll sizes (overridable at build time for tuning sweeps).
+#ifndef CTBB_SCATTER_CH
+#define CTBB_SCATTER_CH 32
+#endif
+#ifndef CTBB_SCATTER_VIEWS
+#define CTBB_SCATTER_VIEWS 2
+#endif
+#ifndef CTBB_ITEM_UNROLL
+#define CTBB_ITEM_UNROLL 4
+#endif
+constexpr int kScatterCh = CTBB_SCATTER_CH;
+constexpr int kScatterViews = CTBB_SCATTER_VIEWS;
 constexpr int kWaveCh = kScatterCh / kWaves;
-constexpr int kSliceUnroll = 4;
+constexpr int kItemUnroll = CTBB_ITEM_UNROLL;
 constexpr uint32_t kSlotBits = 14, kRowBits = 8;
 
 __device__ __forceinline__ uint32_t bt_slot(uint32_t p) { return p & ((1u << kSlotBits) - 1); }
@@ -560,9 +571,8 @@ struct BtArgs {
     int64_t nsm, nch, nr, readings;
 };
 
-// One 64-entry chunk [p0, hi) of slice v: reduce runs of equal samples and
-// add them into the workgroup tile (acc, rows of nr samples per channel
-// starting at channel c0).
+// One 64-entry chunk of an item: reduce runs of equal samples and add them
+// into the view's tile (acc: rows of nr samples per channel from c0).
 __device__ __forceinline__ void scatter_chunk(uint32_t e0, float d, bool ok,
                                               int64_t c0, int nr, int lane,
                                               float *acc)
@@ -574,34 +584,58 @@ __device__ __forceinline__ void scatter_chunk(uint32_t e0, float d, bool ok,
         acc[key] += d;
 }
 
-// LDS: kScatterCh * nr floats (dynamic).
+// Dispatch order: the groups of kScatterViews data views from vlo are
+// visited in residue classes of P (group g, g + P, g + 2P, ...), P = view
+// distance of consecutive members of the slice group / kScatterViews: those
+// groups read the same SSM views' transposed entries (for successive
+// members), so the entries are fetched into L2 once per class; logical
+// workgroups go to the XCDs in contiguous ranges.
+struct ScatterOrder {
+    int64_t vlo, vhi, nq, ncg, P, rows;
+};
+
+// LDS: kScatterViews * kScatterCh * nr floats (dynamic).
 __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
-    SweepArgs a, StepArgs sa, BtArgs bt, int64_t b, int64_t vlo, int64_t ncg)
+    SweepArgs a, StepArgs sa, BtArgs bt, int64_t b, ScatterOrder so)
 {
     extern __shared__ float acc[];
     __shared__ int any_work;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t view = vlo + blockIdx.x / ncg;
-    const int64_t c0 = (blockIdx.x % ncg) * kScatterCh;
+    const int64_t nwg = so.P * so.rows * so.ncg;
+    const int64_t per_xcd = (nwg + kXcd - 1) / kXcd;
+    const int64_t L = (int64_t)(blockIdx.x % kXcd) * per_xcd + blockIdx.x / kXcd;
+    if (L >= nwg)
+        return;
+    const int64_t ql = L / so.ncg;
+    const int64_t cls = ql / so.rows, m = ql - cls * so.rows;
+    const int64_t q = cls + m * so.P;
+    if (q >= so.nq)
+        return;
+    const int64_t view0 = so.vlo + q * kScatterViews;
+    const int64_t nview = min((int64_t)kScatterViews, so.vhi - view0);
+    const int64_t c0 = (L - ql * so.ncg) * kScatterCh;
     const int64_t c1 = min(c0 + kScatterCh, bt.nch);
     const int64_t wc0 = min(c0 + kWaveCh * wave, c1), wc1 = min(wc0 + kWaveCh, c1);
     const int nr = (int)bt.nr;
-    const int nsamp = (int)((c1 - c0) * nr);
+    const int tile = (int)((c1 - c0) * nr);
     const int64_t nvc = bt.nsm * bt.nch + 1;
     const int32_t *vci = bt.vc_idx + b * nvc;
     const uint32_t *ent = bt.ent + bt.ent_off[b];
     const float *val = bt.val + bt.ent_off[b];
     if (tid == 0)
         any_work = 0;
-    for (int i = tid; i < nsamp; i += kSweepBlock)
+    for (int i = tid; i < kScatterViews * tile; i += kSweepBlock)
         acc[i] = 0.0f;
     __syncthreads();
-    for (int64_t vb = 0; vb < sa.ng; vb += 64) {
-        // lane l: entry range of slice vb + l on this wavefront's channels
-        const int64_t vl = vb + lane;
+    // items: lane l of a round covers view u = l / kItemLanes and member
+    // vb + l % kItemLanes
+    constexpr int kItemLanes = 64 / kScatterViews;
+    for (int64_t vb = 0; vb < sa.ng; vb += kItemLanes) {
+        const int u_l = lane / kItemLanes;
+        const int64_t v_l = vb + (lane % kItemLanes);
         int64_t lo = 0, hi = 0;
-        if (vl < sa.ng && wc1 > wc0) {
-            const int64_t i = view - bt.si_half[sa.j + a.spacing * vl];
+        if (u_l < nview && v_l < sa.ng && wc1 > wc0) {
+            const int64_t i = view0 + u_l - bt.si_half[sa.j + a.spacing * v_l];
             if (i >= 0 && i < bt.nsm) {
                 lo = vci[i * bt.nch + wc0];
                 hi = vci[i * bt.nch + wc1];
@@ -611,44 +645,45 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
         if (todo && lane == 0)
             any_work = 1;
         while (todo) {
-            int sl[kSliceUnroll];
-            int64_t lo_u[kSliceUnroll], hi_u[kSliceUnroll];
+            int it[kItemUnroll];
+            int64_t lo_u[kItemUnroll], hi_u[kItemUnroll];
 #pragma unroll
-            for (int u = 0; u < kSliceUnroll; u++) {
-                sl[u] = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
+            for (int u = 0; u < kItemUnroll; u++) {
+                it[u] = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
                 todo &= todo ? todo - 1 : 0;
-                lo_u[u] = __shfl(lo, sl[u] < 0 ? 0 : sl[u], 64);
-                hi_u[u] = sl[u] < 0 ? lo_u[u] : __shfl(hi, sl[u], 64);
+                lo_u[u] = __shfl(lo, it[u] < 0 ? 0 : it[u], 64);
+                hi_u[u] = it[u] < 0 ? lo_u[u] : __shfl(hi, it[u], 64);
             }
-            uint32_t ev[kSliceUnroll];
-            float av[kSliceUnroll];
-            bool ok[kSliceUnroll];
+            uint32_t ev[kItemUnroll];
+            float av[kItemUnroll];
+            bool ok[kItemUnroll];
 #pragma unroll
-            for (int u = 0; u < kSliceUnroll; u++) {
+            for (int u = 0; u < kItemUnroll; u++) {
                 const int64_t p = lo_u[u] + lane;
-                ok[u] = p < hi_u[u] && lane < 64;
+                ok[u] = p < hi_u[u];
                 ev[u] = ok[u] ? ent[p] : 0u;
                 av[u] = ok[u] ? val[p] : 0.0f;
             }
-            float dv[kSliceUnroll];
+            float dv[kItemUnroll];
 #pragma unroll
-            for (int u = 0; u < kSliceUnroll; u++) {
-                const float *du = a.du + (vb + (sl[u] < 0 ? 0 : sl[u])) * sa.nc;
-                dv[u] = ok[u] ? du[bt_slot(ev[u])] * av[u] : 0.0f;
+            for (int u = 0; u < kItemUnroll; u++) {
+                const int64_t vm = vb + (it[u] < 0 ? 0 : (it[u] % kItemLanes));
+                dv[u] = ok[u] ? a.du[vm * sa.nc + bt_slot(ev[u])] * av[u] : 0.0f;
             }
 #pragma unroll
-            for (int u = 0; u < kSliceUnroll; u++) {
-                if (sl[u] < 0)
+            for (int u = 0; u < kItemUnroll; u++) {
+                if (it[u] < 0)
                     continue;
-                scatter_chunk(ev[u], dv[u], ok[u], c0, nr, lane, acc);
-                // slices with more than 64 entries on these channels
-                const float *du = a.du + (vb + sl[u]) * sa.nc;
+                float *tacc = acc + (it[u] / kItemLanes) * tile;
+                scatter_chunk(ev[u], dv[u], ok[u], c0, nr, lane, tacc);
+                // items with more than 64 entries on these channels
+                const float *du = a.du + (vb + (it[u] % kItemLanes)) * sa.nc;
                 for (int64_t p0 = lo_u[u] + 64; p0 < hi_u[u]; p0 += 64) {
                     const int64_t p = p0 + lane;
                     const bool okp = p < hi_u[u];
                     const uint32_t e0 = okp ? ent[p] : 0u;
                     const float d = okp ? du[bt_slot(e0)] * val[p] : 0.0f;
-                    scatter_chunk(e0, d, okp, c0, nr, lane, acc);
+                    scatter_chunk(e0, d, okp, c0, nr, lane, tacc);
                 }
             }
         }
@@ -656,23 +691,26 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
     __syncthreads();
     if (!any_work)
         return;
-    const int64_t r0 = (view * bt.nch + c0) * bt.nr;
-    float *e = a.e + r0;
-    if (a.w) {
-        float *ew = a.ew + r0;
-        const float *w = a.w + r0;
-        for (int i = tid; i < nsamp; i += kSweepBlock) {
-            const float d = acc[i];
-            if (d != 0.0f) {
-                e[i] = e[i] - d;
-                ew[i] = ew[i] - w[i] * d;
+    for (int64_t u = 0; u < nview; u++) {
+        const int64_t r0 = ((view0 + u) * bt.nch + c0) * bt.nr;
+        const float *tacc = acc + u * tile;
+        float *e = a.e + r0;
+        if (a.w) {
+            float *ew = a.ew + r0;
+            const float *w = a.w + r0;
+            for (int i = tid; i < tile; i += kSweepBlock) {
+                const float d = tacc[i];
+                if (d != 0.0f) {
+                    e[i] = e[i] - d;
+                    ew[i] = ew[i] - w[i] * d;
+                }
             }
-        }
-    } else {
-        for (int i = tid; i < nsamp; i += kSweepBlock) {
-            const float d = acc[i];
-            if (d != 0.0f)
-                e[i] = e[i] - d;
+        } else {
+            for (int i = tid; i < tile; i += kSweepBlock) {
+                const float d = tacc[i];
+                if (d != 0.0f)
+                    e[i] = e[i] - d;
+            }
         }
     }
 }
@@ -1560,8 +1598,8 @@ static int build_transposed_batches(ctbb_recon *r, const std::vector<int32_t> &c
     const ctbb_geom *g = &r->g;
     const int64_t nbatch = r->nbatch, ncol = g->nx * g->ny;
     const int64_t nvc = g->n_views_sm * g->n_channels + 1;
-    if (g->n_rows > 256 || g->n_channels > 1024)
-        return fail(CTBB_E_UNSUPPORTED, "transposed batches need n_rows <= 256 "
+    if (g->n_rows > 128 || g->n_channels > 1024)
+        return fail(CTBB_E_UNSUPPORTED, "transposed batches need n_rows <= 128 "
                                         "and n_channels <= 1024");
     if (r->max_batch > (1 << 14))
         return fail(CTBB_E_ARG, "batches of more than 16384 columns (tile too "
@@ -1820,6 +1858,17 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
         r->th2_valid = false;
         if (r->nib > 0xffff || ng_max > 0xffff)
             return bail(fail(CTBB_E_UNSUPPORTED, "too many view blocks or slices per group"));
+        // scatter dispatch stride: median view distance of consecutive members
+        r->view_stride.assign(std::max<int64_t>(1, r->spacing), 1);
+        for (int64_t j = 0; j < r->spacing && j < g->nz; j++) {
+            std::vector<int64_t> d;
+            for (int64_t k = j; k + r->spacing < g->nz; k += r->spacing)
+                d.push_back(std::llabs(r->h_si_half[k + r->spacing] - r->h_si_half[k]));
+            if (d.empty())
+                continue;
+            std::nth_element(d.begin(), d.begin() + d.size() / 2, d.end());
+            r->view_stride[j] = std::max<int64_t>(1, d[d.size() / 2]);
+        }
         std::vector<uint32_t> pairs;
         r->h_pair_off.assign(1, 0);
         for (int64_t j = 0; j < r->spacing; j++) {
@@ -1957,7 +2006,7 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
     bt.nr = r->g.n_rows;
     bt.readings = r->g.readings;
     const int64_t ncg = (r->g.n_channels + kScatterCh - 1) / kScatterCh;
-    const size_t scatter_lds = sizeof(float) * kScatterCh * r->g.n_rows;
+    const size_t scatter_lds = sizeof(float) * kScatterViews * kScatterCh * r->g.n_rows;
 
     hipEvent_t e0, e1;
     CTBB_HIP(hipEventCreate(&e0));
@@ -2020,10 +2069,23 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                                    dim3(kSweepBlock), 0, r->stream, a, sa, ga);
                 if (samp)
                     CTBB_HIP(hipEventRecord(ev[4 * used + 2], r->stream));
-                if (vhi > vlo)
-                    hipLaunchKernelGGL(icd_scatter_kernel, dim3((unsigned)((vhi - vlo) * ncg)),
+                if (vhi > vlo) {
+                    ScatterOrder so;
+                    so.vlo = vlo;
+                    so.vhi = vhi;
+                    so.nq = (vhi - vlo + kScatterViews - 1) / kScatterViews;
+                    so.ncg = ncg;
+                    so.P = std::min<int64_t>(
+                        std::max<int64_t>(1, (r->view_stride[j] + kScatterViews / 2) /
+                                                 kScatterViews),
+                        so.nq);
+                    so.rows = (so.nq + so.P - 1) / so.P;
+                    const int64_t nwg = so.P * so.rows * ncg;
+                    hipLaunchKernelGGL(icd_scatter_kernel,
+                                       dim3((unsigned)(kXcd * ((nwg + kXcd - 1) / kXcd))),
                                        dim3(kSweepBlock), scatter_lds, r->stream, a, sa, bt, b,
-                                       vlo, ncg);
+                                       so);
+                }
                 if (samp) {
                     CTBB_HIP(hipEventRecord(ev[4 * used + 3], r->stream));
                     used++;

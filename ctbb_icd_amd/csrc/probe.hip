@@ -40,13 +40,19 @@ extern "C" int ctbb_copy_probe(int64_t bytes, double *ms)
     if (e == hipSuccess) {
         hipLaunchKernelGGL(ctbb_fill_kernel, dim3(8192), dim3(256), 0, 0, a, n);
         hipLaunchKernelGGL(ctbb_fill_kernel, dim3(8192), dim3(256), 0, 0, b, n);
-        e = hipEventRecord(e0, 0);
-        hipLaunchKernelGGL(ctbb_copy_kernel, dim3(8192), dim3(256), 0, 0, a, b, n);
-        if (e == hipSuccess) e = hipEventRecord(e1, 0);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float t = 0.0f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-        if (ms) *ms = t;
+        // best of five copies (the first launches also warm clocks and TLBs)
+        float best = 0.0f;
+        for (int rep = 0; rep < 5 && e == hipSuccess; rep++) {
+            e = hipEventRecord(e0, 0);
+            hipLaunchKernelGGL(ctbb_copy_kernel, dim3(8192), dim3(256), 0, 0, a, b, n);
+            if (e == hipSuccess) e = hipEventRecord(e1, 0);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float t = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+            if (rep == 0 || t < best)
+                best = t;
+        }
+        if (ms) *ms = best;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
