@@ -57,6 +57,20 @@ def copy_probe(ct, mib: int) -> None:
           f"written in {ms:.3f} ms ({2 * mib / 1024 / ms * 1e3:.0f} GiB/s)", file=sys.stderr)
 
 
+def measured_traffic(workload_key: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by
+    what the counter reads for a copy of known size), or None."""
+    f = ROOT / "profiles" / f"icd_sweep_{workload_key}_counters.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def cpu_baseline(scan, ssm, raw, w, target_s: float, threads: int):
     """Time the reference-structured OpenMP ICD (oracle) on a band of rows."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -173,7 +187,9 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": dom,
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": (measured_traffic(args.config, dom)
+                        if not args.fp16 and args.tile in (0, 8) else None),
             "algorithmic_bytes_per_launch": dom_bytes,
             "avg_launch_ms": dom_ms,
         },
