@@ -128,6 +128,16 @@ def lib() -> C.CDLL:
         "ctbb_recon_get_error": ([P, P], C.c_int),
         "ctbb_rotate_slices": ([C.POINTER(Geom), P, P, P, C.c_int], C.c_int),
         "ctbb_penalty_steps": ([C.POINTER(Geom), C.POINTER(IcdOpts), C.c_int, P, _i64, P, P, P, P], C.c_int),
+        "ctbb_unet_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
+                              C.POINTER(P)], C.c_int),
+        "ctbb_unet_free": ([P], C.c_int),
+        "ctbb_unet_num_params": ([P, C.POINTER(_i64)], C.c_int),
+        "ctbb_unet_get_params": ([P, P], C.c_int),
+        "ctbb_unet_set_params": ([P, P], C.c_int),
+        "ctbb_unet_grad": ([P, P, P, C.c_int, C.POINTER(_f64), P], C.c_int),
+        "ctbb_unet_adam_step": ([P, _f64, _f64, _f64, _f64], C.c_int),
+        "ctbb_unet_infer": ([P, P, C.c_int, P], C.c_int),
+        "ctbb_unet_last_timing": ([P, C.POINTER(_f64), C.POINTER(_f64)], C.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -368,6 +378,71 @@ def penalty_steps(geom: Geom, opts: IcdOpts, x, q, alpha, beta, solver: int = 0)
     check(lib().ctbb_penalty_steps(C.byref(geom), C.byref(opts), solver, _p(x), len(q), _p(q),
                                    _p(a), _p(b), _p(du)))
     return du
+
+
+class UNet:
+    """The warm-start network (include/ctbb_unet.h): BF16 MFMA convolutions,
+    FP32 master weights and Adam, all on the device."""
+
+    def __init__(self, levels: int = 4, base: int = 32, max_batch: int = 8,
+                 height: int = 256, width: int = 256, seed: int = 1):
+        self.levels, self.base, self.max_batch = levels, base, max_batch
+        self.height, self.width = height, width
+        h = C.c_void_p()
+        check(lib().ctbb_unet_create(levels, base, max_batch, height, width, seed,
+                                     C.byref(h)))
+        self.h = h
+        n = _i64()
+        check(lib().ctbb_unet_num_params(self.h, C.byref(n)))
+        self.num_params = n.value
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, np.float32)
+        check(lib().ctbb_unet_get_params(self.h, _p(p)))
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.num_params
+        check(lib().ctbb_unet_set_params(self.h, _p(p)))
+
+    def grad(self, inp, target, want_grads: bool = False):
+        inp = np.ascontiguousarray(inp, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        b = inp.shape[0]
+        assert inp.shape == (b, 3, self.height, self.width)
+        assert target.shape == (b, self.height, self.width)
+        loss = C.c_double()
+        g = np.zeros(self.num_params, np.float32) if want_grads else None
+        check(lib().ctbb_unet_grad(self.h, _p(inp), _p(target), b, C.byref(loss), _p(g)))
+        return (loss.value, g) if want_grads else loss.value
+
+    def adam_step(self, lr: float, beta1: float = 0.9, beta2: float = 0.999,
+                  eps: float = 1e-8) -> None:
+        check(lib().ctbb_unet_adam_step(self.h, lr, beta1, beta2, eps))
+
+    def infer(self, inp) -> np.ndarray:
+        inp = np.ascontiguousarray(inp, np.float32)
+        b = inp.shape[0]
+        out = np.zeros((b, self.height, self.width), np.float32)
+        check(lib().ctbb_unet_infer(self.h, _p(inp), b, _p(out)))
+        return out
+
+    def last_timing(self):
+        ms, fl = C.c_double(), C.c_double()
+        check(lib().ctbb_unet_last_timing(self.h, C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value
+
+    def free(self) -> None:
+        if getattr(self, "h", None):
+            lib().ctbb_unet_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def copy_probe(nbytes: int) -> float:

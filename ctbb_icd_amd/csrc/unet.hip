@@ -1,0 +1,981 @@
+// unet.hip -- learned warm start: a 2.5D U-Net trained and run on gfx950.
+//
+// No counterpart in the reference (it supplies the start image that
+// initialize_recon_volume, src/initialize_recon_volume.cpp:51-76, reads from
+// a file or sets to zero).  Network, parameter order and layouts: see
+// include/ctbb_unet.h.
+//
+// 3x3 convolutions are implicit GEMMs on MFMA (v_mfma_f32_32x32x8_bf16):
+//   forward        out[p][co] = b[co] + sum_{tap,ci} in[p + o(tap)][ci] W[tap][co][ci]
+//                  one wavefront per 32 pixels x 32 output channels
+//   input grad     a forward convolution of dY with Wt[tap][ci][co] =
+//                  W[8 - tap][co][ci] (flipped taps, swapped channels)
+//   weight grad    dW[tap][co][ci] = sum_p dY[p][co] in[p + o(tap)][ci], split
+//                  over pixel chunks (four wavefronts per chunk, combined in
+//                  LDS in wavefront order), chunks added in order by a
+//                  second kernel: deterministic
+// MFMA operand layout (32x32x8, one k-group of 4 per half-wavefront): lane l
+// holds A[l % 32][4 (l / 32) + j] and B[4 (l / 32) + j][l % 32], j = 0..3; the
+// 16 results of lane l are D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+// Activations and weights are BF16 (round to nearest even), every sum FP32.
+#include "common.hpp"
+
+#include "../../include/ctbb_unet.h"
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+using namespace ctbb;
+
+namespace {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__host__ __device__ inline uint16_t f2bf(float f)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u)
+        return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float bf2f(uint16_t b)
+{
+    return __builtin_bit_cast(float, (uint32_t)b << 16);
+}
+
+// Input of a convolution: tensor a (ca channels) followed by tensor b (cb
+// channels, optionally at half resolution, read with nearest upsampling).
+struct ConvIn {
+    const uint16_t *a;
+    int ca;
+    const uint16_t *b;
+    int cb;
+    int up;
+};
+
+__device__ inline v4s load_in4(const ConvIn &in, int n, int y, int x, int H, int W, int c)
+{
+    v4s z = {0, 0, 0, 0};
+    if (y < 0 || y >= H || x < 0 || x >= W)
+        return z;
+    if (c < in.ca)
+        return *reinterpret_cast<const v4s *>(in.a + ((((int64_t)n * H + y) * W + x) * in.ca + c));
+    c -= in.ca;
+    if (in.up) {
+        const int H2 = H >> 1, W2 = W >> 1;
+        return *reinterpret_cast<const v4s *>(
+            in.b + ((((int64_t)n * H2 + (y >> 1)) * W2 + (x >> 1)) * in.cb + c));
+    }
+    return *reinterpret_cast<const v4s *>(in.b + ((((int64_t)n * H + y) * W + x) * in.cb + c));
+}
+
+__device__ inline uint16_t load_in1(const ConvIn &in, int n, int y, int x, int H, int W, int c)
+{
+    if (y < 0 || y >= H || x < 0 || x >= W)
+        return 0;
+    if (c < in.ca)
+        return in.a[(((int64_t)n * H + y) * W + x) * in.ca + c];
+    c -= in.ca;
+    if (in.up) {
+        const int H2 = H >> 1, W2 = W >> 1;
+        return in.b[(((int64_t)n * H2 + (y >> 1)) * W2 + (x >> 1)) * in.cb + c];
+    }
+    return in.b[(((int64_t)n * H + y) * W + x) * in.cb + c];
+}
+
+// Forward 3x3 convolution (zero padding), + bias, optional ReLU, BF16 out.
+// cin must be a multiple of 8.  Grid: 4 tiles (wavefronts) per workgroup,
+// tile = (32-pixel block, 32-channel block).
+__global__ __launch_bounds__(256) void conv3x3_fwd_kernel(ConvIn in, const uint16_t *w,
+                                                          const float *bias, uint16_t *out,
+                                                          int N, int H, int W, int cin,
+                                                          int cout, int relu, int nct)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t tile = (int64_t)blockIdx.x * 4 + wave;
+    const int64_t P = (int64_t)N * H * W;
+    const int64_t pt = tile / nct;
+    const int ct = (int)(tile - pt * nct);
+    if (pt * 32 >= P)
+        return;
+    const int r = lane & 31, kg = lane >> 5;
+    const int64_t p = pt * 32 + r;
+    const bool pv = p < P;
+    int n = 0, y = 0, x = 0;
+    if (pv) {
+        n = (int)(p / ((int64_t)H * W));
+        const int rem = (int)(p - (int64_t)n * H * W);
+        y = rem / W;
+        x = rem - y * W;
+    }
+    const int co = ct * 32 + r;
+    const bool cv = co < cout;
+    v16f acc = {0};
+    const v4s z = {0, 0, 0, 0};
+    for (int tap = 0; tap < 9; tap++) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const uint16_t *wt = w + ((int64_t)tap * cout + (cv ? co : 0)) * cin + 4 * kg;
+        for (int ci = 0; ci < cin; ci += 8) {
+            const v4s av = pv ? load_in4(in, n, y + dy, x + dx, H, W, ci + 4 * kg) : z;
+            const v4s bv = cv ? *reinterpret_cast<const v4s *>(wt + ci) : z;
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+        }
+    }
+    const int oc = ct * 32 + r;
+    if (oc >= cout)
+        return;
+    const float bb = bias ? bias[oc] : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int i = 8 * (q >> 2) + 4 * kg + (q & 3);
+        const int64_t pix = pt * 32 + i;
+        if (pix < P) {
+            float v = acc[q] + bb;
+            if (relu && v < 0.0f)
+                v = 0.0f;
+            out[pix * cout + oc] = f2bf(v);
+        }
+    }
+}
+
+// Weight gradient partials: part[chunk][tap][co][ci] = sum over the chunk's
+// pixels of dY[p][co] * in[p + o(tap)][ci].  Four wavefronts take quarters
+// of the chunk; their 32x32 tiles are added in LDS in wavefront order.
+__global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(ConvIn in, const uint16_t *dy,
+                                                            int N, int H, int W, int cin,
+                                                            int cout, int64_t chunk,
+                                                            int nco, int nci, float *part)
+{
+    __shared__ float red[4][16][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int64_t b = blockIdx.x;
+    const int ci_t = (int)(b % nci);
+    b /= nci;
+    const int co_t = (int)(b % nco);
+    b /= nco;
+    const int tap = (int)(b % 9);
+    const int64_t ck = b / 9;
+    const int dyo = tap / 3 - 1, dxo = tap % 3 - 1;
+    const int64_t P = (int64_t)N * H * W;
+    const int64_t q4 = chunk / 4;
+    const int64_t p0 = ck * chunk + wave * q4;
+    const int64_t p1 = min(p0 + q4, P);
+    const int r = lane & 31, kg = lane >> 5;
+    const int co = co_t * 32 + r, ci = ci_t * 32 + r;
+    const bool cov = co < cout, civ = ci < cin;
+    v16f acc = {0};
+    for (int64_t p = p0; p < p1; p += 8) {
+        v4s av, bv;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t pp = p + 4 * kg + j;
+            uint16_t a = 0, bb = 0;
+            if (pp < p1) {
+                if (cov)
+                    a = dy[pp * cout + co];
+                if (civ) {
+                    const int n = (int)(pp / ((int64_t)H * W));
+                    const int rem = (int)(pp - (int64_t)n * H * W);
+                    const int yy = rem / W, xx = rem - (rem / W) * W;
+                    bb = load_in1(in, n, yy + dyo, xx + dxo, H, W, ci);
+                }
+            }
+            av[j] = (short)a;
+            bv[j] = (short)bb;
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        red[wave][q][lane] = acc[q];
+    __syncthreads();
+    if (wave != 0)
+        return;
+    const int cc = ci_t * 32 + r;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int i = 8 * (q >> 2) + 4 * kg + (q & 3);
+        const int oc = co_t * 32 + i;
+        if (oc < cout && cc < cin) {
+            const float s = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) +
+                            red[3][q][lane];
+            part[(((int64_t)ck * 9 + tap) * cout + oc) * cin + cc] = s;
+        }
+    }
+}
+
+// g[i] = sum over chunks of part[chunk][i], in chunk order.
+__global__ void sum_chunks_kernel(const float *part, int64_t nchunk, int64_t n, float *g)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int64_t c = 0; c < nchunk; c++)
+            s += part[c * n + i];
+        g[i] = s;
+    }
+}
+
+// db[co] = sum_p dY[p][co]: one workgroup per channel, fixed tree order.
+__global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t *dy, int64_t P,
+                                                        int cout, float *db)
+{
+    __shared__ float sh[256];
+    const int co = blockIdx.x;
+    float s = 0.0f;
+    for (int64_t p = threadIdx.x; p < P; p += 256)
+        s += bf2f(dy[p * cout + co]);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        db[co] = sh[0];
+}
+
+__global__ void maxpool_fwd_kernel(const uint16_t *in, uint16_t *out, int N, int H, int W,
+                                   int C)
+{
+    const int H2 = H >> 1, W2 = W >> 1;
+    const int64_t n_out = (int64_t)N * H2 * W2 * C;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_out;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        int64_t t = i / C;
+        const int x = (int)(t % W2);
+        t /= W2;
+        const int y = (int)(t % H2);
+        const int n = (int)(t / H2);
+        float m = -INFINITY;
+        for (int dy = 0; dy < 2; dy++)
+            for (int dx = 0; dx < 2; dx++)
+                m = fmaxf(m, bf2f(in[(((int64_t)n * H + 2 * y + dy) * W + 2 * x + dx) * C + c]));
+        out[i] = f2bf(m);
+    }
+}
+
+// dIn = routes dOut to the first maximum of each 2x2 window (ties: row-major
+// first), everything else 0.  Writes every element of dIn.
+__global__ void maxpool_bwd_kernel(const uint16_t *in, const uint16_t *dout, uint16_t *din,
+                                   int N, int H, int W, int C)
+{
+    const int H2 = H >> 1, W2 = W >> 1;
+    const int64_t n_out = (int64_t)N * H2 * W2 * C;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_out;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        int64_t t = i / C;
+        const int x = (int)(t % W2);
+        t /= W2;
+        const int y = (int)(t % H2);
+        const int n = (int)(t / H2);
+        float m = -INFINITY;
+        int best = 0;
+        for (int k = 0; k < 4; k++) {
+            const float v =
+                bf2f(in[(((int64_t)n * H + 2 * y + k / 2) * W + 2 * x + k % 2) * C + c]);
+            if (v > m) {
+                m = v;
+                best = k;
+            }
+        }
+        for (int k = 0; k < 4; k++)
+            din[(((int64_t)n * H + 2 * y + k / 2) * W + 2 * x + k % 2) * C + c] =
+                k == best ? dout[i] : (uint16_t)0;
+    }
+}
+
+// dY[i] = 0 where the ReLU output Y[i] is 0.
+__global__ void relu_mask_kernel(const uint16_t *y, uint16_t *dy, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (!(bf2f(y[i]) > 0.0f))
+            dy[i] = 0;
+}
+
+// Gradient of a concatenated input: the first ca channels are added to dA
+// (in FP32, rounded to BF16); the cb upsampled channels are summed over their
+// 2x2 source block into dB (half resolution), in a fixed order.
+__global__ void split_concat_grad_kernel(const uint16_t *dx, int N, int H, int W, int ca,
+                                         int cb, uint16_t *da, uint16_t *db)
+{
+    const int64_t na = (int64_t)N * H * W * ca;
+    const int H2 = H >> 1, W2 = W >> 1;
+    const int64_t nb = (int64_t)N * H2 * W2 * cb;
+    const int C = ca + cb;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < na) {
+            const int c = (int)(i % ca);
+            const int64_t p = i / ca;
+            da[i] = f2bf(bf2f(da[i]) + bf2f(dx[p * C + c]));
+        } else {
+            const int64_t j = i - na;
+            const int c = (int)(j % cb);
+            int64_t t = j / cb;
+            const int x = (int)(t % W2);
+            t /= W2;
+            const int y = (int)(t % H2);
+            const int n = (int)(t / H2);
+            float s = 0.0f;
+            for (int k = 0; k < 4; k++)
+                s += bf2f(dx[(((int64_t)n * H + 2 * y + k / 2) * W + 2 * x + k % 2) * C + ca + c]);
+            db[j] = f2bf(s);
+        }
+    }
+}
+
+// Input packing: host layout [B][3][H][W] FP32 -> NHWC BF16 with 8 channels
+// (3 slices, 5 zeros) and the FP32 centre slice [B][H][W].
+__global__ void pack_input_kernel(const float *src, int N, int H, int W, uint16_t *x8,
+                                  float *centre)
+{
+    const int64_t P = (int64_t)N * H * W;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = p / ((int64_t)H * W), hw = p - n * H * W;
+        for (int c = 0; c < 8; c++)
+            x8[p * 8 + c] = c < 3 ? f2bf(src[(n * 3 + c) * H * W + hw]) : (uint16_t)0;
+        centre[p] = src[(n * 3 + 1) * H * W + hw];
+    }
+}
+
+// Final 1x1 convolution + residual: out[p] = centre[p] + b + sum_c w[c] f[p][c].
+__global__ void final_fwd_kernel(const uint16_t *f, const float *w, const float *b, int C,
+                                 const float *centre, int64_t P, float *out)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        float s = b[0];
+        for (int c = 0; c < C; c++)
+            s += w[c] * bf2f(f[p * C + c]);
+        out[p] = centre[p] + s;
+    }
+}
+
+// g[p] = 2 (out - t) / P and per-block partial sums of (out - t)^2.
+__global__ __launch_bounds__(256) void loss_grad_kernel(const float *out, const float *t,
+                                                        int64_t P, float *g, double *partials)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < P;
+         p += (int64_t)gridDim.x * 256) {
+        const float d = out[p] - t[p];
+        g[p] = 2.0f * d / (float)P;
+        s += (double)d * d;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = sh[0];
+}
+
+// Backward of the final layer: df[p][c] = g[p] w[c] where f[p][c] > 0 (the
+// ReLU of the last 3x3 convolution), and per-block partials of
+// dw[c] = sum_p g[p] f[p][c], db = sum_p g[p] ([nblk][C + 1]).
+__global__ __launch_bounds__(256) void final_bwd_kernel(const uint16_t *f, const float *w,
+                                                        int C, const float *g, int64_t P,
+                                                        uint16_t *df, float *partials)
+{
+    extern __shared__ float shf[];   // [256][C + 1]
+    for (int c = 0; c <= C; c++)
+        shf[threadIdx.x * (C + 1) + c] = 0.0f;
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < P;
+         p += (int64_t)gridDim.x * 256) {
+        const float gp = g[p];
+        for (int c = 0; c < C; c++) {
+            const float fv = bf2f(f[p * C + c]);
+            df[p * C + c] = fv > 0.0f ? f2bf(gp * w[c]) : (uint16_t)0;
+            shf[threadIdx.x * (C + 1) + c] += gp * fv;
+        }
+        shf[threadIdx.x * (C + 1) + C] += gp;
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int c = 0; c <= C; c++)
+                shf[threadIdx.x * (C + 1) + c] += shf[(threadIdx.x + o) * (C + 1) + c];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c <= C; c++)
+            partials[(int64_t)blockIdx.x * (C + 1) + c] = shf[c];
+}
+
+__global__ void adam_kernel(float *w, float *m, float *v, const float *g, int64_t n,
+                            float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// BF16 copies of a convolution's weights: wbf = W, wt[tap][ci][co] =
+// W[8 - tap][co][ci] (the input-gradient convolution).
+__global__ void refresh_weights_kernel(const float *w, int cin, int cout, uint16_t *wbf,
+                                       uint16_t *wt)
+{
+    const int64_t n = 9LL * cin * cout;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int ci = (int)(i % cin);
+        const int64_t t = i / cin;
+        const int co = (int)(t % cout);
+        const int tap = (int)(t / cout);
+        const uint16_t b = f2bf(w[i]);
+        wbf[i] = b;
+        if (wt)
+            wt[((int64_t)(8 - tap) * cin + ci) * cout + co] = b;
+    }
+}
+
+int grid_for(int64_t n)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 16384));
+}
+
+struct Conv {
+    int cin, cout, level;      // level: spatial size H >> level
+    int64_t woff, boff;        // parameter offsets
+    uint16_t *wbf = nullptr;   // [9][cout][cin]
+    uint16_t *wt = nullptr;    // [9][cin][cout] (null for the first layer)
+    ConvIn in{};
+    uint16_t *out = nullptr, *dout = nullptr;
+};
+
+}  // namespace
+
+struct ctbb_unet {
+    int L = 4, base = 32, max_batch = 1, H = 0, W = 0;
+    std::vector<Conv> convs;           // encoder 0..L-1 (a, b), decoder L-2..0 (a, b)
+    std::vector<uint16_t *> pool, dpool;   // [L-1] pooled encoder outputs and grads
+    std::vector<uint16_t *> dskip;     // [L-1] concat gradient of the encoder outputs
+    int64_t nparams = 0, final_w = 0, final_b = 0;
+    float *params = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+    uint16_t *x8 = nullptr, *dxfull = nullptr, *dfinal = nullptr;
+    float *centre = nullptr, *outp = nullptr, *target = nullptr, *gout = nullptr,
+          *host_in = nullptr, *wpart = nullptr, *fpart = nullptr;
+    double *lpart = nullptr;
+    int64_t wpart_n = 0;
+    int step = 0;
+    double last_ms = 0.0, last_flops = 0.0;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+int ualloc(ctbb_unet *u, void **p, size_t bytes, const char *what)
+{
+    CTBB_TRY(dev_alloc(p, std::max<size_t>(bytes, 16), what));
+    u->allocs.push_back(*p);
+    return CTBB_OK;
+}
+
+template <typename T>
+int ualloc_n(ctbb_unet *u, T **p, size_t n, const char *what)
+{
+    return ualloc(u, reinterpret_cast<void **>(p), n * sizeof(T), what);
+}
+
+int64_t pixels(const ctbb_unet *u, int batch, int level)
+{
+    return (int64_t)batch * (u->H >> level) * (u->W >> level);
+}
+
+int run_conv_fwd(ctbb_unet *u, const Conv &c, int batch, double *flops)
+{
+    const int Hl = u->H >> c.level, Wl = u->W >> c.level;
+    const int64_t P = (int64_t)batch * Hl * Wl;
+    const int nct = (c.cout + 31) / 32;
+    const int64_t tiles = ((P + 31) / 32) * nct;
+    hipLaunchKernelGGL(conv3x3_fwd_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0,
+                       u->stream, c.in, c.wbf, u->params + c.boff, c.out, batch, Hl, Wl,
+                       c.cin, c.cout, 1, nct);
+    *flops += 2.0 * P * 9.0 * c.cin * c.cout;
+    CTBB_HIP(hipGetLastError());
+    return CTBB_OK;
+}
+
+// Input gradient of conv c into dst ([P][cin]), from c.dout (masked).
+int run_conv_dgrad(ctbb_unet *u, const Conv &c, int batch, uint16_t *dst, double *flops)
+{
+    const int Hl = u->H >> c.level, Wl = u->W >> c.level;
+    const int64_t P = (int64_t)batch * Hl * Wl;
+    ConvIn gi{c.dout, c.cout, nullptr, 0, 0};
+    const int nct = (c.cin + 31) / 32;
+    const int64_t tiles = ((P + 31) / 32) * nct;
+    hipLaunchKernelGGL(conv3x3_fwd_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0,
+                       u->stream, gi, c.wt, nullptr, dst, batch, Hl, Wl, c.cout, c.cin, 0,
+                       nct);
+    *flops += 2.0 * P * 9.0 * c.cin * c.cout;
+    CTBB_HIP(hipGetLastError());
+    return CTBB_OK;
+}
+
+constexpr int64_t kWgradChunk = 4096;
+
+int run_conv_wgrad(ctbb_unet *u, const Conv &c, int batch, double *flops)
+{
+    const int Hl = u->H >> c.level, Wl = u->W >> c.level;
+    const int64_t P = (int64_t)batch * Hl * Wl;
+    const int64_t nchunk = (P + kWgradChunk - 1) / kWgradChunk;
+    const int nco = (c.cout + 31) / 32, nci = (c.cin + 31) / 32;
+    const int64_t n = 9LL * c.cout * c.cin;
+    if (nchunk * n > u->wpart_n)
+        return fail(CTBB_E_STATE, "weight-gradient scratch too small");
+    hipLaunchKernelGGL(conv3x3_wgrad_kernel, dim3((unsigned)(nchunk * 9 * nco * nci)),
+                       dim3(256), 0, u->stream, c.in, c.dout, batch, Hl, Wl, c.cin, c.cout,
+                       kWgradChunk, nco, nci, u->wpart);
+    hipLaunchKernelGGL(sum_chunks_kernel, dim3(grid_for(n)), dim3(256), 0, u->stream,
+                       u->wpart, nchunk, n, u->grad + c.woff);
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(c.cout), dim3(256), 0, u->stream, c.dout, P,
+                       c.cout, u->grad + c.boff);
+    *flops += 2.0 * P * 9.0 * c.cin * c.cout;
+    CTBB_HIP(hipGetLastError());
+    return CTBB_OK;
+}
+
+int refresh_bf16(ctbb_unet *u)
+{
+    for (auto &c : u->convs)
+        hipLaunchKernelGGL(refresh_weights_kernel, dim3(grid_for(9LL * c.cin * c.cout)),
+                           dim3(256), 0, u->stream, u->params + c.woff, c.cin, c.cout, c.wbf,
+                           c.wt);
+    CTBB_HIP(hipGetLastError());
+    return CTBB_OK;
+}
+
+void mask(ctbb_unet *u, const uint16_t *y, uint16_t *dy, int64_t n)
+{
+    hipLaunchKernelGGL(relu_mask_kernel, dim3(grid_for(n)), dim3(256), 0, u->stream, y, dy, n);
+}
+
+// Forward pass over `batch` packed inputs (u->x8, u->centre) into u->outp.
+int forward(ctbb_unet *u, int batch, double *flops)
+{
+    const int L = u->L;
+    for (int l = 0; l < L; l++) {
+        Conv &a = u->convs[2 * l], &b = u->convs[2 * l + 1];
+        CTBB_TRY(run_conv_fwd(u, a, batch, flops));
+        CTBB_TRY(run_conv_fwd(u, b, batch, flops));
+        if (l < L - 1) {
+            const int Hl = u->H >> l, Wl = u->W >> l;
+            hipLaunchKernelGGL(maxpool_fwd_kernel,
+                               dim3(grid_for(pixels(u, batch, l + 1) * b.cout)), dim3(256), 0,
+                               u->stream, b.out, u->pool[l], batch, Hl, Wl, b.cout);
+        }
+    }
+    for (int d = 0; d < L - 1; d++) {   // decoder position d: level L-2-d
+        Conv &a = u->convs[2 * L + 2 * d], &b = u->convs[2 * L + 2 * d + 1];
+        CTBB_TRY(run_conv_fwd(u, a, batch, flops));
+        CTBB_TRY(run_conv_fwd(u, b, batch, flops));
+    }
+    const Conv &last = u->convs.back();
+    hipLaunchKernelGGL(final_fwd_kernel, dim3(grid_for(pixels(u, batch, 0))), dim3(256), 0,
+                       u->stream, last.out, u->params + u->final_w, u->params + u->final_b,
+                       u->base, u->centre, pixels(u, batch, 0), u->outp);
+    CTBB_HIP(hipGetLastError());
+    return CTBB_OK;
+}
+
+int backward(ctbb_unet *u, int batch, double *flops)
+{
+    const int L = u->L;
+    const int64_t P0 = pixels(u, batch, 0);
+    // final layer
+    const int nblk = (int)std::min<int64_t>((P0 + 255) / 256, 1024);
+    Conv &last = u->convs.back();
+    hipLaunchKernelGGL(final_bwd_kernel, dim3(nblk), dim3(256),
+                       256 * (u->base + 1) * sizeof(float), u->stream, last.out,
+                       u->params + u->final_w, u->base, u->gout, P0, last.dout, u->fpart);
+    hipLaunchKernelGGL(sum_chunks_kernel, dim3(1), dim3(256), 0, u->stream, u->fpart,
+                       (int64_t)nblk, (int64_t)(u->base + 1), u->grad + u->final_w);
+    CTBB_HIP(hipGetLastError());
+    // decoder positions d = L-2 .. 0 (position d works at level L-2-d): the
+    // reverse of the forward order
+    for (int d = L - 2; d >= 0; d--) {
+        Conv &a = u->convs[2 * L + 2 * d], &b = u->convs[2 * L + 2 * d + 1];
+        // b: dout already holds the masked gradient
+        CTBB_TRY(run_conv_wgrad(u, b, batch, flops));
+        CTBB_TRY(run_conv_dgrad(u, b, batch, a.dout, flops));
+        mask(u, a.out, a.dout, pixels(u, batch, a.level) * a.cout);
+        CTBB_TRY(run_conv_wgrad(u, a, batch, flops));
+        CTBB_TRY(run_conv_dgrad(u, a, batch, u->dxfull, flops));
+        // split: skip part -> dskip[level], upsampled part -> producer's dout
+        const int lvl = a.level;
+        const Conv &prod = (d == 0) ? u->convs[2 * (L - 1) + 1]
+                                    : u->convs[2 * L + 2 * (d - 1) + 1];
+        CTBB_HIP(hipMemsetAsync(u->dskip[lvl], 0,
+                                pixels(u, batch, lvl) * a.in.ca * sizeof(uint16_t), u->stream));
+        const int64_t nsplit = pixels(u, batch, lvl) * a.in.ca +
+                               pixels(u, batch, lvl + 1) * a.in.cb;
+        hipLaunchKernelGGL(split_concat_grad_kernel, dim3(grid_for(nsplit)), dim3(256), 0,
+                           u->stream, u->dxfull, batch, u->H >> lvl, u->W >> lvl, a.in.ca,
+                           a.in.cb, u->dskip[lvl], prod.dout);
+        mask(u, prod.out, prod.dout, pixels(u, batch, prod.level) * prod.cout);
+        CTBB_HIP(hipGetLastError());
+    }
+    // encoder, deepest level first
+    for (int l = L - 1; l >= 0; l--) {
+        Conv &a = u->convs[2 * l], &b = u->convs[2 * l + 1];
+        if (l < L - 1) {
+            // dout of b = pool gradient routed back + concat (skip) gradient
+            const int Hl = u->H >> l, Wl = u->W >> l;
+            hipLaunchKernelGGL(maxpool_bwd_kernel,
+                               dim3(grid_for(pixels(u, batch, l + 1) * b.cout)), dim3(256), 0,
+                               u->stream, b.out, u->dpool[l], b.dout, batch, Hl, Wl, b.cout);
+            hipLaunchKernelGGL(split_concat_grad_kernel,
+                               dim3(grid_for(pixels(u, batch, l) * b.cout)), dim3(256), 0,
+                               u->stream, u->dskip[l], batch, Hl, Wl, b.cout, 0, b.dout,
+                               nullptr);
+            mask(u, b.out, b.dout, pixels(u, batch, l) * b.cout);
+        }
+        CTBB_TRY(run_conv_wgrad(u, b, batch, flops));
+        CTBB_TRY(run_conv_dgrad(u, b, batch, a.dout, flops));
+        mask(u, a.out, a.dout, pixels(u, batch, l) * a.cout);
+        CTBB_TRY(run_conv_wgrad(u, a, batch, flops));
+        if (l > 0)
+            CTBB_TRY(run_conv_dgrad(u, a, batch, u->dpool[l - 1], flops));
+        CTBB_HIP(hipGetLastError());
+    }
+    return CTBB_OK;
+}
+
+int upload_inputs(ctbb_unet *u, const float *input, int batch)
+{
+    const int64_t P = pixels(u, batch, 0);
+    CTBB_HIP(hipMemcpyAsync(u->host_in, input, P * 3 * sizeof(float), hipMemcpyHostToDevice,
+                            u->stream));
+    hipLaunchKernelGGL(pack_input_kernel, dim3(grid_for(P)), dim3(256), 0, u->stream,
+                       u->host_in, batch, u->H, u->W, u->x8, u->centre);
+    CTBB_HIP(hipGetLastError());
+    return CTBB_OK;
+}
+
+}  // namespace
+
+extern "C" int ctbb_unet_create(int levels, int base_channels, int max_batch, int height,
+                                int width, uint64_t seed, ctbb_unet **out)
+{
+    if (!out)
+        return fail(CTBB_E_ARG, "null argument");
+    *out = nullptr;
+    if (levels < 2 || levels > 6 || base_channels < 8 || base_channels % 8 || max_batch < 1)
+        return fail(CTBB_E_ARG, "need 2 <= levels <= 6, base_channels a multiple of 8, "
+                                "max_batch >= 1");
+    const int m = 1 << (levels - 1);
+    if (height < m || width < m || height % m || width % m)
+        return fail(CTBB_E_ARG, "height and width must be multiples of 2^(levels-1)");
+    ctbb_unet *u = new ctbb_unet();
+    u->L = levels;
+    u->base = base_channels;
+    u->max_batch = max_batch;
+    u->H = height;
+    u->W = width;
+    auto bail = [&](int rc) {
+        ctbb_unet_free(u);
+        return rc;
+    };
+    hipError_t he = hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking);
+    if (he != hipSuccess)
+        return bail(fail(CTBB_E_HIP, std::string("stream: ") + hipGetErrorString(he)));
+    const int L = levels;
+    std::vector<int> ch(L);
+    for (int l = 0; l < L; l++)
+        ch[l] = base_channels << l;
+    // layer table in parameter order
+    int64_t off = 0;
+    auto add = [&](int cin, int cout, int level) {
+        Conv c;
+        c.cin = cin;
+        c.cout = cout;
+        c.level = level;
+        c.woff = off;
+        off += 9LL * cin * cout;
+        c.boff = off;
+        off += cout;
+        u->convs.push_back(c);
+    };
+    for (int l = 0; l < L; l++) {
+        add(l == 0 ? 8 : ch[l - 1], ch[l], l);
+        add(ch[l], ch[l], l);
+    }
+    for (int d = L - 2; d >= 0; d--) {
+        add(ch[d] + ch[d + 1], ch[d], d);
+        add(ch[d], ch[d], d);
+    }
+    u->final_w = off;
+    off += base_channels;
+    u->final_b = off;
+    off += 1;
+    u->nparams = off;
+    int rc = CTBB_OK;
+    if ((rc = ualloc_n(u, &u->params, off, "U-Net parameters")) ||
+        (rc = ualloc_n(u, &u->grad, off, "U-Net gradients")) ||
+        (rc = ualloc_n(u, &u->m, off, "Adam moments")) ||
+        (rc = ualloc_n(u, &u->v, off, "Adam moments")))
+        return bail(rc);
+    const int64_t P0 = pixels(u, max_batch, 0);
+    // activations and gradients
+    for (auto &c : u->convs) {
+        const int64_t n = pixels(u, max_batch, c.level) * c.cout;
+        if ((rc = ualloc_n(u, &c.out, n, "U-Net activations")) ||
+            (rc = ualloc_n(u, &c.dout, n, "U-Net activation gradients")) ||
+            (rc = ualloc_n(u, &c.wbf, 9LL * c.cin * c.cout, "U-Net BF16 weights")))
+            return bail(rc);
+        if (&c != &u->convs[0] &&
+            (rc = ualloc_n(u, &c.wt, 9LL * c.cin * c.cout, "U-Net BF16 weights")))
+            return bail(rc);
+    }
+    u->pool.assign(L - 1, nullptr);
+    u->dpool.assign(L - 1, nullptr);
+    u->dskip.assign(L - 1, nullptr);
+    for (int l = 0; l < L - 1; l++) {
+        if ((rc = ualloc_n(u, &u->pool[l], pixels(u, max_batch, l + 1) * ch[l], "pooled")) ||
+            (rc = ualloc_n(u, &u->dpool[l], pixels(u, max_batch, l + 1) * ch[l], "pooled")) ||
+            (rc = ualloc_n(u, &u->dskip[l], pixels(u, max_batch, l) * ch[l], "skip grads")))
+            return bail(rc);
+    }
+    int64_t dx_max = 0, wp_max = 0;
+    for (auto &c : u->convs) {
+        dx_max = std::max(dx_max, pixels(u, max_batch, c.level) * c.cin);
+        const int64_t nch = (pixels(u, max_batch, c.level) + kWgradChunk - 1) / kWgradChunk;
+        wp_max = std::max<int64_t>(wp_max, nch * 9 * (int64_t)c.cin * c.cout);
+    }
+    u->wpart_n = wp_max;
+    if ((rc = ualloc_n(u, &u->x8, P0 * 8, "packed input")) ||
+        (rc = ualloc_n(u, &u->dxfull, dx_max, "input gradients")) ||
+        (rc = ualloc_n(u, &u->centre, P0, "centre slices")) ||
+        (rc = ualloc_n(u, &u->outp, P0, "output")) ||
+        (rc = ualloc_n(u, &u->target, P0, "target")) ||
+        (rc = ualloc_n(u, &u->gout, P0, "output gradient")) ||
+        (rc = ualloc_n(u, &u->host_in, P0 * 3, "input staging")) ||
+        (rc = ualloc_n(u, &u->wpart, wp_max, "weight-gradient partials")) ||
+        (rc = ualloc_n(u, &u->fpart, 1024LL * (base_channels + 1), "final partials")) ||
+        (rc = ualloc_n(u, &u->lpart, 1024, "loss partials")))
+        return bail(rc);
+    // wire inputs
+    for (int l = 0; l < L; l++) {
+        Conv &a = u->convs[2 * l], &b = u->convs[2 * l + 1];
+        a.in = l == 0 ? ConvIn{u->x8, 8, nullptr, 0, 0}
+                      : ConvIn{u->pool[l - 1], ch[l - 1], nullptr, 0, 0};
+        b.in = ConvIn{a.out, a.cout, nullptr, 0, 0};
+    }
+    for (int d = L - 2; d >= 0; d--) {
+        const int i = 2 * L + 2 * (L - 2 - d);
+        Conv &a = u->convs[i], &b = u->convs[i + 1];
+        const Conv &below = (d == L - 2) ? u->convs[2 * (L - 1) + 1] : u->convs[i - 1];
+        a.in = ConvIn{u->convs[2 * d + 1].out, ch[d], below.out, ch[d + 1], 1};
+        b.in = ConvIn{a.out, a.cout, nullptr, 0, 0};
+    }
+    // He initialisation (normal, std sqrt(2 / fan_in)), biases 0, final 0
+    std::vector<float> h(off, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (auto &c : u->convs) {
+        std::normal_distribution<double> nd(0.0, std::sqrt(2.0 / (9.0 * c.cin)));
+        for (int64_t i = 0; i < 9LL * c.cin * c.cout; i++) {
+            const int ci = (int)(i % c.cin);
+            // zero padding channels of the input layer carry no signal
+            h[c.woff + i] = (c.woff == 0 && ci >= 3) ? 0.0f : (float)nd(rng);
+        }
+    }
+    he = hipMemcpy(u->params, h.data(), off * sizeof(float), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemset(u->m, 0, off * sizeof(float));
+    if (he == hipSuccess) he = hipMemset(u->v, 0, off * sizeof(float));
+    if (he == hipSuccess) he = hipMemset(u->grad, 0, off * sizeof(float));
+    if (he != hipSuccess)
+        return bail(fail(CTBB_E_HIP, std::string("U-Net init: ") + hipGetErrorString(he)));
+    if ((rc = refresh_bf16(u)) != CTBB_OK)
+        return bail(rc);
+    he = hipStreamSynchronize(u->stream);
+    if (he != hipSuccess)
+        return bail(fail(CTBB_E_HIP, std::string("U-Net init: ") + hipGetErrorString(he)));
+    *out = u;
+    return CTBB_OK;
+}
+
+extern "C" int ctbb_unet_free(ctbb_unet *u)
+{
+    if (!u)
+        return CTBB_OK;
+    if (u->stream)
+        (void)hipStreamSynchronize(u->stream);
+    for (void *p : u->allocs)
+        (void)hipFree(p);
+    if (u->stream)
+        (void)hipStreamDestroy(u->stream);
+    delete u;
+    return CTBB_OK;
+}
+
+extern "C" int ctbb_unet_num_params(const ctbb_unet *u, int64_t *n)
+{
+    if (!u || !n)
+        return fail(CTBB_E_ARG, "null argument");
+    *n = u->nparams;
+    return CTBB_OK;
+}
+
+extern "C" int ctbb_unet_get_params(const ctbb_unet *u, float *params)
+{
+    if (!u || !params)
+        return fail(CTBB_E_ARG, "null argument");
+    CTBB_HIP(hipStreamSynchronize(u->stream));
+    CTBB_HIP(hipMemcpy(params, u->params, u->nparams * sizeof(float), hipMemcpyDeviceToHost));
+    return CTBB_OK;
+}
+
+extern "C" int ctbb_unet_set_params(ctbb_unet *u, const float *params)
+{
+    if (!u || !params)
+        return fail(CTBB_E_ARG, "null argument");
+    CTBB_HIP(hipStreamSynchronize(u->stream));
+    CTBB_HIP(hipMemcpy(u->params, params, u->nparams * sizeof(float), hipMemcpyHostToDevice));
+    CTBB_TRY(refresh_bf16(u));
+    CTBB_HIP(hipStreamSynchronize(u->stream));
+    return CTBB_OK;
+}
+
+extern "C" int ctbb_unet_grad(ctbb_unet *u, const float *input, const float *target,
+                              int batch, double *loss, float *grads)
+{
+    if (!u || !input || !target)
+        return fail(CTBB_E_ARG, "null argument");
+    if (batch < 1 || batch > u->max_batch)
+        return fail(CTBB_E_ARG, "batch must be in [1, max_batch]");
+    hipEvent_t e0, e1;
+    CTBB_HIP(hipEventCreate(&e0));
+    CTBB_HIP(hipEventCreate(&e1));
+    const int64_t P0 = pixels(u, batch, 0);
+    double flops = 0.0;
+    int rc = upload_inputs(u, input, batch);
+    if (rc == CTBB_OK) {
+        hipError_t he = hipMemcpyAsync(u->target, target, P0 * sizeof(float),
+                                       hipMemcpyHostToDevice, u->stream);
+        if (he != hipSuccess)
+            rc = fail(CTBB_E_HIP, std::string("target upload: ") + hipGetErrorString(he));
+    }
+    if (rc == CTBB_OK && hipEventRecord(e0, u->stream) != hipSuccess)
+        rc = fail(CTBB_E_HIP, "event record");
+    if (rc == CTBB_OK)
+        rc = forward(u, batch, &flops);
+    const int nblk = (int)std::min<int64_t>((P0 + 255) / 256, 1024);
+    if (rc == CTBB_OK) {
+        hipLaunchKernelGGL(loss_grad_kernel, dim3(nblk), dim3(256), 0, u->stream, u->outp,
+                           u->target, P0, u->gout, u->lpart);
+        rc = backward(u, batch, &flops);
+    }
+    std::vector<double> lp(nblk);
+    if (rc == CTBB_OK) {
+        hipError_t he = hipEventRecord(e1, u->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(lp.data(), u->lpart, nblk * sizeof(double),
+                                hipMemcpyDeviceToHost, u->stream);
+        if (he == hipSuccess && grads)
+            he = hipMemcpyAsync(grads, u->grad, u->nparams * sizeof(float),
+                                hipMemcpyDeviceToHost, u->stream);
+        if (he == hipSuccess)
+            he = hipStreamSynchronize(u->stream);
+        float ms = 0.0f;
+        if (he == hipSuccess)
+            he = hipEventElapsedTime(&ms, e0, e1);
+        if (he != hipSuccess)
+            rc = fail(CTBB_E_HIP, std::string("U-Net step: ") + hipGetErrorString(he));
+        u->last_ms = ms;
+        u->last_flops = flops;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != CTBB_OK)
+        return rc;
+    double s = 0.0;
+    for (double x : lp)
+        s += x;
+    if (loss)
+        *loss = s / (double)P0;
+    return CTBB_OK;
+}
+
+extern "C" int ctbb_unet_adam_step(ctbb_unet *u, double lr, double beta1, double beta2,
+                                   double eps)
+{
+    if (!u)
+        return fail(CTBB_E_ARG, "null argument");
+    u->step++;
+    const float c1 = (float)(1.0 - std::pow(beta1, u->step));
+    const float c2 = (float)(1.0 - std::pow(beta2, u->step));
+    hipLaunchKernelGGL(adam_kernel, dim3(grid_for(u->nparams)), dim3(256), 0, u->stream,
+                       u->params, u->m, u->v, u->grad, u->nparams, (float)lr, (float)beta1,
+                       (float)beta2, (float)eps, c1, c2);
+    CTBB_HIP(hipGetLastError());
+    CTBB_TRY(refresh_bf16(u));
+    return CTBB_OK;
+}
+
+extern "C" int ctbb_unet_infer(ctbb_unet *u, const float *input, int batch, float *output)
+{
+    if (!u || !input || !output)
+        return fail(CTBB_E_ARG, "null argument");
+    if (batch < 1 || batch > u->max_batch)
+        return fail(CTBB_E_ARG, "batch must be in [1, max_batch]");
+    hipEvent_t e0, e1;
+    CTBB_HIP(hipEventCreate(&e0));
+    CTBB_HIP(hipEventCreate(&e1));
+    double flops = 0.0;
+    int rc = upload_inputs(u, input, batch);
+    if (rc == CTBB_OK && hipEventRecord(e0, u->stream) != hipSuccess)
+        rc = fail(CTBB_E_HIP, "event record");
+    if (rc == CTBB_OK)
+        rc = forward(u, batch, &flops);
+    if (rc == CTBB_OK) {
+        hipError_t he = hipEventRecord(e1, u->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(output, u->outp, pixels(u, batch, 0) * sizeof(float),
+                                hipMemcpyDeviceToHost, u->stream);
+        if (he == hipSuccess)
+            he = hipStreamSynchronize(u->stream);
+        float ms = 0.0f;
+        if (he == hipSuccess)
+            he = hipEventElapsedTime(&ms, e0, e1);
+        if (he != hipSuccess)
+            rc = fail(CTBB_E_HIP, std::string("U-Net inference: ") + hipGetErrorString(he));
+        u->last_ms = ms;
+        u->last_flops = flops;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+extern "C" int ctbb_unet_last_timing(const ctbb_unet *u, double *ms, double *conv_flops)
+{
+    if (!u)
+        return fail(CTBB_E_ARG, "null argument");
+    if (ms)
+        *ms = u->last_ms;
+    if (conv_flops)
+        *conv_flops = u->last_flops;
+    return CTBB_OK;
+}
