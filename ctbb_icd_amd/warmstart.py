@@ -1,0 +1,283 @@
+"""Learned warm start for the ICD reconstruction.
+
+The start image of initialize_recon_volume (src/initialize_recon_volume.cpp:
+51-76,180-183: zeros or a file) is replaced by the output of a 2.5D U-Net
+(include/ctbb_unet.h) applied to the image after a few plain ICD
+iterations.  The network works in the fixed frame: the rotated slices are
+de-rotated (ctbb_rotate_slices, rotating -> fixed), stacked as (k-1, k, k+1)
+with the end slices repeated, scaled by 1/mu_water, mapped, scaled back,
+clamped at 0 and rotated back.  The ICD cost decides: a warm start whose
+FP64 cost exceeds that of the plain image it came from is rejected and the
+plain image is used instead.
+
+Training pairs come from the GPU ICD on seeded phantoms of one scan
+geometry: (image after `n_early` iterations from zero, image after
+`n_target` iterations).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Recon, SSM, Scan, UNet, rotate_slices
+
+MU_WATER = 0.193   # 1/cm; inputs and outputs are scaled by 1/MU_WATER
+
+
+def slice_stacks(vol: np.ndarray) -> np.ndarray:
+    """[nz, ny, nx] -> [nz, 3, ny, nx]: slices (k-1, k, k+1), ends repeated."""
+    prev = np.concatenate([vol[:1], vol[:-1]])
+    nxt = np.concatenate([vol[1:], vol[-1:]])
+    return np.stack([prev, vol, nxt], axis=1)
+
+
+def to_fixed(scan: Scan, x_rot: np.ndarray) -> np.ndarray:
+    g = scan.geom
+    v = rotate_slices(g, scan.tube_angles, scan.slice_index, x_rot, +1)
+    return v.reshape(g.nz, g.ny, g.nx)
+
+
+def to_rotating(scan: Scan, vol: np.ndarray) -> np.ndarray:
+    g = scan.geom
+    return rotate_slices(g, scan.tube_angles, scan.slice_index,
+                         np.ascontiguousarray(vol, np.float32).reshape(-1), -1)
+
+
+def predict_fixed(net: UNet, vol: np.ndarray) -> np.ndarray:
+    """Network output for every slice of a fixed-frame volume [nz, ny, nx]."""
+    inp = (slice_stacks(vol) / MU_WATER).astype(np.float32)
+    out = np.empty(vol.shape, np.float32)
+    for k0 in range(0, vol.shape[0], net.max_batch):
+        k1 = min(k0 + net.max_batch, vol.shape[0])
+        out[k0:k1] = net.infer(inp[k0:k1])
+    return np.maximum(out * MU_WATER, 0.0)
+
+
+@dataclass
+class Pair:
+    seed: int
+    early: np.ndarray     # fixed frame [nz, ny, nx]
+    target: np.ndarray
+
+
+def generate_pairs(make_scan, seeds, ssm: SSM, n_early: int, n_target: int,
+                   log=print) -> list[Pair]:
+    """make_scan(seed) -> Scan of the shared geometry with that phantom."""
+    pairs = []
+    for s in seeds:
+        t = time.perf_counter()
+        scan = make_scan(s)
+        raw, w = scan.synthesize(use_gpu=True)
+        r = Recon(ssm, raw, w, scan.slice_index, scan.opts())
+        r.iterate(n_early)
+        early = to_fixed(scan, r.volume())
+        r.iterate(n_target - n_early)
+        target = to_fixed(scan, r.volume())
+        r.free()
+        pairs.append(Pair(s, early, target))
+        log(f"[warmstart] pair seed {s}: {time.perf_counter() - t:.1f} s")
+    return pairs
+
+
+def train(net: UNet, pairs: list[Pair], epochs: int, lr: float, seed: int = 0,
+          log=print) -> list[float]:
+    """Adam on the mean squared error (scaled units), slices shuffled; cosine
+    learning-rate decay to lr/20.  Returns the mean loss of every epoch."""
+    rng = np.random.default_rng(seed)
+    inp = np.concatenate([slice_stacks(p.early) for p in pairs]) / MU_WATER
+    tgt = np.concatenate([p.target for p in pairs]) / MU_WATER
+    inp = inp.astype(np.float32)
+    tgt = tgt.astype(np.float32)
+    n = inp.shape[0]
+    steps = epochs * ((n + net.max_batch - 1) // net.max_batch)
+    hist, k = [], 0
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        losses = []
+        for b0 in range(0, n, net.max_batch):
+            sel = order[b0:b0 + net.max_batch]
+            losses.append(net.grad(inp[sel], tgt[sel]))
+            frac = k / max(1, steps - 1)
+            net.adam_step(lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * frac))))
+            k += 1
+        hist.append(float(np.mean(losses)))
+        log(f"[warmstart] epoch {ep + 1}/{epochs}: loss {hist[-1]:.4e}")
+    return hist
+
+
+@dataclass
+class Run:
+    """One reconstruction to a cost target."""
+    start: str
+    iterations: int = 0
+    seconds: float = 0.0
+    reached: bool = False
+    costs: list = field(default_factory=list)
+    accepted: bool | None = None    # warm starts: passed the cost check
+
+
+def run_to_target(r: Recon, target: float, max_iter: int, t0: float, run: Run,
+                  log=print) -> Run:
+    """Iterate until the FP64 cost is <= target (checked after every
+    iteration; the cost evaluation is not counted in the time)."""
+    spent = time.perf_counter() - t0
+    while run.iterations < max_iter:
+        t = time.perf_counter()
+        r.iterate(1)
+        spent += time.perf_counter() - t
+        run.iterations += 1
+        c = r.cost()
+        run.costs.append(c)
+        if c <= target:
+            run.reached = True
+            break
+    run.seconds = spent
+    log(f"[warmstart] {run.start}: {run.iterations} iterations, {run.seconds:.2f} s, "
+        f"reached={run.reached}")
+    return run
+
+
+def plain_run(scan: Scan, ssm: SSM, raw, w, target: float, max_iter: int,
+              log=print) -> Run:
+    r = Recon(ssm, raw, w, scan.slice_index, scan.opts())
+    try:
+        return run_to_target(r, target, max_iter, time.perf_counter(), Run("plain"), log)
+    finally:
+        r.free()
+
+
+def warm_run(scan: Scan, ssm: SSM, raw, w, net: UNet, n_early: int, target: float,
+             max_iter: int, log=print) -> Run:
+    """n_early plain iterations, the network, the cost check, then ICD to the
+    target.  The time includes the early iterations, the rotations, the
+    network, the projection of the start image and the two cost evaluations
+    of the check; the iteration count includes the early iterations."""
+    run = Run("warm")
+    r = Recon(ssm, raw, w, scan.slice_index, scan.opts())
+    try:
+        t0 = time.perf_counter()
+        r.iterate(n_early)
+        x_plain = r.volume()
+        x_warm = to_rotating(scan, predict_fixed(net, to_fixed(scan, x_plain)))
+        c_plain = r.cost()
+        r.set_volume(x_warm)
+        c_warm = r.cost()
+        run.accepted = bool(c_warm <= c_plain)
+        if not run.accepted:
+            r.set_volume(x_plain)
+        run.iterations = n_early
+        run.costs = [c_warm if run.accepted else c_plain]
+        log(f"[warmstart] start cost plain {c_plain:.8e} warm {c_warm:.8e} -> "
+            f"{'accepted' if run.accepted else 'rejected'}")
+        if run.costs[0] <= target:
+            run.reached = True
+            run.seconds = time.perf_counter() - t0
+            return run
+        return run_to_target(r, target, max_iter, t0, run, log)
+    finally:
+        r.free()
+
+
+def _main():
+    import argparse
+    import json
+    from pathlib import Path
+    from . import configs, init, set_verbose
+
+    ap = argparse.ArgumentParser(description="train the warm-start network and "
+                                 "measure time to cost tolerance, plain vs warm")
+    ap.add_argument("--config", default="config3")
+    ap.add_argument("--train", type=int, default=16, help="training phantoms")
+    ap.add_argument("--test", type=int, default=3, help="held-out phantoms")
+    ap.add_argument("--early", type=int, default=1, help="plain iterations before the network")
+    ap.add_argument("--target-its", type=int, default=40, help="iterations of the training targets")
+    ap.add_argument("--ref-its", type=int, default=80, help="iterations of the reference runs")
+    ap.add_argument("--tol", type=float, default=1e-4, help="relative cost tolerance")
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--fp16", action="store_true", help="FP16 SSM for every run")
+    ap.add_argument("--workdir", default="/tmp/ctbb_warmstart")
+    ap.add_argument("--out", default="", help="JSON summary path")
+    ap.add_argument("--params-out", default="", help="trained parameters (.npy)")
+    a = ap.parse_args()
+
+    set_verbose(False)
+    init(0)
+    wd = Path(a.workdir)
+    wd.mkdir(parents=True, exist_ok=True)
+
+    def make_scan(seed):
+        return Scan(configs.write_prm(a.config, wd / f"{a.config}_{seed}.yaml",
+                                      synthetic_seed=seed))
+
+    def log(msg):
+        print(msg, flush=True)
+
+    s0 = make_scan(1)
+    g = s0.geom
+    t = time.perf_counter()
+    ssm = SSM.build(g)
+    if a.fp16:
+        ssm.to_fp16()
+    log(f"[warmstart] {a.config}: {g.nx}x{g.ny}x{g.nz}, SSM in {time.perf_counter() - t:.1f} s")
+    pairs = generate_pairs(make_scan, range(1000, 1000 + a.train), ssm, a.early,
+                           a.target_its, log)
+    net = UNet(4, 32, a.batch, g.ny, g.nx, seed=11)
+    t = time.perf_counter()
+    hist = train(net, pairs, a.epochs, a.lr, log=log)
+    t_train = time.perf_counter() - t
+    ms, flops = net.last_timing()
+    log(f"[warmstart] training {t_train:.1f} s; last step {ms:.2f} ms, "
+        f"{flops / (ms * 1e-3) / 1e12:.1f} TFLOP/s in the 3x3 convolutions")
+    if a.params_out:
+        np.save(a.params_out, net.params())
+    results = []
+    for s in range(2000, 2000 + a.test):
+        scan = make_scan(s)
+        raw, w = scan.synthesize(use_gpu=True)
+        # reference: plain ICD for ref-its iterations; target = its final cost
+        ref = Run("reference")
+        r = Recon(ssm, raw, w, scan.slice_index, scan.opts())
+        t0 = time.perf_counter()
+        secs = []
+        for _ in range(a.ref_its):
+            t = time.perf_counter()
+            r.iterate(1)
+            secs.append(time.perf_counter() - t)
+            ref.costs.append(r.cost())
+        r.free()
+        target = ref.costs[-1] * (1 + a.tol)
+        n_plain = next(i + 1 for i, c in enumerate(ref.costs) if c <= target)
+        plain = Run("plain", iterations=n_plain, seconds=float(np.sum(secs[:n_plain])),
+                    reached=True, costs=ref.costs[:n_plain])
+        log(f"[warmstart] seed {s}: converged cost {ref.costs[-1]:.10e}, plain reaches "
+            f"{a.tol:g} after {n_plain} iterations ({plain.seconds:.2f} s)")
+        warm = warm_run(scan, ssm, raw, w, net, a.early, target, a.ref_its, log)
+        results.append(dict(seed=s, converged_cost=ref.costs[-1], target=target,
+                            plain_iterations=plain.iterations, plain_seconds=plain.seconds,
+                            warm_iterations=warm.iterations, warm_seconds=warm.seconds,
+                            warm_reached=warm.reached, warm_accepted=warm.accepted,
+                            plain_costs=plain.costs, warm_costs=warm.costs))
+    summary = dict(config=a.config, fp16=a.fp16, train=a.train, test=a.test, early=a.early,
+                   target_its=a.target_its, ref_its=a.ref_its, tol=a.tol, epochs=a.epochs,
+                   train_seconds=t_train, loss_history=hist,
+                   unet_step_ms=ms, unet_step_tflops=flops / (ms * 1e-3) / 1e12,
+                   results=results)
+    pi = float(np.mean([x["plain_iterations"] for x in results]))
+    wi = float(np.mean([x["warm_iterations"] for x in results]))
+    ps = float(np.mean([x["plain_seconds"] for x in results]))
+    ws = float(np.mean([x["warm_seconds"] for x in results]))
+    summary.update(mean_plain_iterations=pi, mean_warm_iterations=wi,
+                   mean_plain_seconds=ps, mean_warm_seconds=ws,
+                   speedup=ps / ws if ws > 0 else None)
+    log(f"[warmstart] mean: plain {pi:.1f} its / {ps:.2f} s, warm {wi:.1f} its / "
+        f"{ws:.2f} s, speedup {ps / ws:.2f}x")
+    if a.out:
+        Path(a.out).write_text(json.dumps(summary, indent=1))
+
+
+if __name__ == "__main__":
+    _main()
