@@ -126,6 +126,7 @@ def lib() -> C.CDLL:
         "ctbb_recon_resync": ([P], C.c_int),
         "ctbb_recon_cost": ([P, P], C.c_int),
         "ctbb_recon_get_error": ([P, P], C.c_int),
+        "ctbb_cost": ([P, P, P, P, C.POINTER(IcdOpts), P, P], C.c_int),
         "ctbb_rotate_slices": ([C.POINTER(Geom), P, P, P, C.c_int], C.c_int),
         "ctbb_penalty_steps": ([C.POINTER(Geom), C.POINTER(IcdOpts), C.c_int, P, _i64, P, P, P, P], C.c_int),
         "ctbb_unet_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
@@ -359,6 +360,17 @@ class Recon:
             self.free()
         except Exception:
             pass
+
+
+def cost(ssm: "SSM", raw, weights, slice_index, opts: IcdOpts, volume) -> float:
+    """FP64 cost of a volume (ctbb_cost), no reconstruction context needed."""
+    raw = np.ascontiguousarray(raw, np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    si = np.ascontiguousarray(slice_index, np.int64)
+    v = np.ascontiguousarray(volume, np.float32)
+    c = C.c_double()
+    check(lib().ctbb_cost(ssm.h, _p(raw), _p(w), _p(si), C.byref(opts), _p(v), C.byref(c)))
+    return c.value
 
 
 def rotate_slices(geom: Geom, tube_angles, slice_index, volume, direction: int) -> np.ndarray:

@@ -16,6 +16,8 @@ the reference's key set.
   config2          config1 with the FP16 SSM and an FP64 resync every 2 its
   config3          config1 geometry cut to 256x256x32 (1.4 mm pixels)
   config4          512x512x256, 6 rotations, lung texture, I0 = 1e5
+  config4_slab     config4's pixels and phantom model on a 32-slice slab
+                   (2 rotations): training pairs for the warm start
 """
 from __future__ import annotations
 
@@ -98,9 +100,13 @@ CONFIGS["micro"] = dict(
 CONFIGS["config2"] = dict(CONFIGS["config1"], ssm_fp16=1, resync_every=2)
 CONFIGS["config3"] = dict(CONFIGS["config1"], Readings=4608, nx=256, ny=256,
                           z_start=-4.0, z_end=-2.05)
+# weights = counts make the data term dominate the penalty: batches of
+# columns 16 apart (as config0) keep the Jacobi coupling of a step small
 CONFIGS["config4"] = dict(CONFIGS["config1"], Readings=13824, z_start=-16.5,
                           z_end=-1.11, synthetic_I0=1e5, synthetic_lung=4000,
-                          synthetic_weights=1)
+                          synthetic_weights=1, tile=16)
+CONFIGS["config4_slab"] = dict(CONFIGS["config4"], Readings=4608, z_start=-4.0,
+                               z_end=-2.05)
 
 
 def prm_text(name: str, **overrides) -> str:

@@ -1939,6 +1939,78 @@ extern "C" int ctbb_recon_resync(ctbb_recon *r)
     return recon_data_term(r, true, &d);
 }
 
+extern "C" int ctbb_cost(const ctbb_ssm *m, const float *raw, const float *weights,
+                         const int64_t *slice_index, const ctbb_icd_opts *opts,
+                         const float *volume, double *cost)
+{
+    if (!m || !raw || !slice_index || !opts || !volume || !cost)
+        return fail(CTBB_E_ARG, "null argument");
+    if (opts->penalty != CTBB_PENALTY_QUADRATIC &&
+        opts->penalty != CTBB_PENALTY_EDGE_PRESERVING)
+        return fail(CTBB_E_ARG, "unknown penalty (valid: quadratic, edge-preserving)");
+    const ctbb_geom *g = &m->g;
+    const int64_t nvox = g->nx * g->ny * g->nz, ds = data_size_of(g);
+    for (int64_t k = 0; k < g->nz; k++)
+        if (slice_index[k] < 0 || slice_index[k] >= g->readings)
+            return fail(CTBB_E_ARG, "slice_index out of range");
+    std::vector<int64_t> sih(g->nz);
+    for (int64_t k = 0; k < g->nz; k++)
+        sih[k] = slice_offset(g, slice_index[k]) / (g->n_channels * g->n_rows);
+    std::vector<int32_t> klo, khi;
+    view_slice_ranges(g, sih, &klo, &khi);
+    float *d_x = nullptr, *d_y = nullptr, *d_w = nullptr;
+    int64_t *d_sih = nullptr;
+    int32_t *d_klo = nullptr, *d_khi = nullptr;
+    double *d_part = nullptr;
+    const int64_t np = std::max<int64_t>(project_grid(g), red_grid(nvox));
+    int rc = CTBB_OK;
+    auto cleanup = [&]() {
+        for (void *p : {(void *)d_x, (void *)d_y, (void *)d_w, (void *)d_sih, (void *)d_klo,
+                        (void *)d_khi, (void *)d_part})
+            if (p) (void)hipFree(p);
+    };
+    if ((rc = dev_alloc_n(&d_x, nvox, "volume")) ||
+        (rc = dev_alloc_n(&d_y, ds, "sinogram")) ||
+        (weights && (rc = dev_alloc_n(&d_w, ds, "weights"))) ||
+        (rc = dev_alloc_n(&d_sih, g->nz, "slice view offsets")) ||
+        (rc = dev_alloc_n(&d_klo, g->readings, "view slice ranges")) ||
+        (rc = dev_alloc_n(&d_khi, g->readings, "view slice ranges")) ||
+        (rc = dev_alloc_n(&d_part, np, "reduction partials"))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemcpy(d_x, volume, nvox * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_y, raw, ds * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && weights) e = hipMemcpy(d_w, weights, ds * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_sih, sih.data(), g->nz * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_klo, klo.data(), g->readings * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_khi, khi.data(), g->readings * 4, hipMemcpyHostToDevice);
+    double data = 0.0, pen = 0.0;
+    if (e == hipSuccess) {
+        rc = launch_project<float, 1>(m, d_x, d_sih, d_klo, d_khi, nullptr, d_y, d_w, nullptr,
+                                      nullptr, d_part, 0);
+        if (rc == CTBB_OK)
+            rc = sum_partials(d_part, (int)project_grid(g), 0, &data);
+        if (rc == CTBB_OK) {
+            const int gp = red_grid(nvox);
+            hipLaunchKernelGGL(penalty_cost_kernel, dim3(gp), dim3(kRedBlock), 0, 0, d_x,
+                               g->nx, nvox, opts->penalty, opts->delta, d_part);
+            e = hipGetLastError();
+            if (e == hipSuccess)
+                rc = sum_partials(d_part, gp, 0, &pen);
+        }
+    }
+    cleanup();
+    if (rc != CTBB_OK)
+        return rc;
+    CTBB_HIP(e);
+    if (opts->penalty == CTBB_PENALTY_QUADRATIC)
+        *cost = 0.5 * data + 0.5 * opts->lambda * pen;
+    else
+        *cost = data + opts->lambda * pen;
+    return CTBB_OK;
+}
+
 extern "C" int ctbb_recon_cost(ctbb_recon *r, double *cost)
 {
     if (!r || !cost)

@@ -169,8 +169,18 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(ConvIn in, const uin
     const int co = co_t * 32 + r, ci = ci_t * 32 + r;
     const bool cov = co < cout, civ = ci < cin;
     v16f acc = {0};
+    // (n, y, x) of this lane's first pixel p0 + 4 kg, advanced incrementally
+    int64_t pq = p0 + 4 * kg;
+    int n0 = 0, y0 = 0, x0 = 0;
+    if (pq < P) {
+        n0 = (int)(pq / ((int64_t)H * W));
+        const int rem = (int)(pq - (int64_t)n0 * H * W);
+        y0 = rem / W;
+        x0 = rem - y0 * W;
+    }
     for (int64_t p = p0; p < p1; p += 8) {
         v4s av, bv;
+        int n = n0, y = y0, x = x0;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int64_t pp = p + 4 * kg + j;
@@ -178,17 +188,29 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(ConvIn in, const uin
             if (pp < p1) {
                 if (cov)
                     a = dy[pp * cout + co];
-                if (civ) {
-                    const int n = (int)(pp / ((int64_t)H * W));
-                    const int rem = (int)(pp - (int64_t)n * H * W);
-                    const int yy = rem / W, xx = rem - (rem / W) * W;
-                    bb = load_in1(in, n, yy + dyo, xx + dxo, H, W, ci);
-                }
+                if (civ)
+                    bb = load_in1(in, n, y + dyo, x + dxo, H, W, ci);
             }
             av[j] = (short)a;
             bv[j] = (short)bb;
+            if (++x == W) {
+                x = 0;
+                if (++y == H) {
+                    y = 0;
+                    n++;
+                }
+            }
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+        // next group: 8 pixels on from this lane's first pixel
+        x0 += 8;
+        while (x0 >= W) {
+            x0 -= W;
+            if (++y0 == H) {
+                y0 = 0;
+                n0++;
+            }
+        }
     }
 #pragma unroll
     for (int q = 0; q < 16; q++)
@@ -221,24 +243,33 @@ __global__ void sum_chunks_kernel(const float *part, int64_t nchunk, int64_t n, 
     }
 }
 
-// db[co] = sum_p dY[p][co]: one workgroup per channel, fixed tree order.
+// Bias gradient partials: part[chunk][co] = sum over the chunk's pixels of
+// dY[p][co].  Threads t of a workgroup take channel t % cout and pixels
+// chunk_start + t / cout, + 256 / cout, ... (coalesced rows of dY); the
+// 256 / cout partial sums of a channel are added in LDS in a fixed order.
+// Needs cout to divide 256.
+constexpr int64_t kBiasChunk = 4096;
+
 __global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t *dy, int64_t P,
-                                                        int cout, float *db)
+                                                        int cout, float *part)
 {
     __shared__ float sh[256];
-    const int co = blockIdx.x;
+    const int t = threadIdx.x;
+    const int c = t % cout, lanes = 256 / cout, l = t / cout;
+    const int64_t p0 = (int64_t)blockIdx.x * kBiasChunk;
+    const int64_t p1 = min(p0 + kBiasChunk, P);
     float s = 0.0f;
-    for (int64_t p = threadIdx.x; p < P; p += 256)
-        s += bf2f(dy[p * cout + co]);
-    sh[threadIdx.x] = s;
+    if (l < lanes)
+        for (int64_t p = p0 + l; p < p1; p += lanes)
+            s += bf2f(dy[p * cout + c]);
+    sh[t] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
+    if (l == 0) {
+        float tot = 0.0f;
+        for (int i = 0; i < lanes; i++)
+            tot += sh[i * cout + c];
+        part[(int64_t)blockIdx.x * cout + c] = tot;
     }
-    if (threadIdx.x == 0)
-        db[co] = sh[0];
 }
 
 __global__ void maxpool_fwd_kernel(const uint16_t *in, uint16_t *out, int N, int H, int W,
@@ -475,7 +506,7 @@ struct ctbb_unet {
     float *params = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
     uint16_t *x8 = nullptr, *dxfull = nullptr, *dfinal = nullptr;
     float *centre = nullptr, *outp = nullptr, *target = nullptr, *gout = nullptr,
-          *host_in = nullptr, *wpart = nullptr, *fpart = nullptr;
+          *host_in = nullptr, *wpart = nullptr, *fpart = nullptr, *bpart = nullptr;
     double *lpart = nullptr;
     int64_t wpart_n = 0;
     int step = 0;
@@ -550,8 +581,11 @@ int run_conv_wgrad(ctbb_unet *u, const Conv &c, int batch, double *flops)
                        kWgradChunk, nco, nci, u->wpart);
     hipLaunchKernelGGL(sum_chunks_kernel, dim3(grid_for(n)), dim3(256), 0, u->stream,
                        u->wpart, nchunk, n, u->grad + c.woff);
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(c.cout), dim3(256), 0, u->stream, c.dout, P,
-                       c.cout, u->grad + c.boff);
+    const int64_t nb = (P + kBiasChunk - 1) / kBiasChunk;
+    hipLaunchKernelGGL(bias_grad_kernel, dim3((unsigned)nb), dim3(256), 0, u->stream, c.dout,
+                       P, c.cout, u->bpart);
+    hipLaunchKernelGGL(sum_chunks_kernel, dim3(grid_for(c.cout)), dim3(256), 0, u->stream,
+                       u->bpart, nb, (int64_t)c.cout, u->grad + c.boff);
     *flops += 2.0 * P * 9.0 * c.cin * c.cout;
     CTBB_HIP(hipGetLastError());
     return CTBB_OK;
@@ -682,9 +716,11 @@ extern "C" int ctbb_unet_create(int levels, int base_channels, int max_batch, in
     if (!out)
         return fail(CTBB_E_ARG, "null argument");
     *out = nullptr;
-    if (levels < 2 || levels > 6 || base_channels < 8 || base_channels % 8 || max_batch < 1)
-        return fail(CTBB_E_ARG, "need 2 <= levels <= 6, base_channels a multiple of 8, "
-                                "max_batch >= 1");
+    if (levels < 2 || levels > 6 || base_channels < 8 ||
+        (base_channels & (base_channels - 1)) || (base_channels << (levels - 1)) > 256 ||
+        max_batch < 1)
+        return fail(CTBB_E_ARG, "need 2 <= levels <= 6, base_channels a power of two >= 8 "
+                                "with base_channels * 2^(levels-1) <= 256, max_batch >= 1");
     const int m = 1 << (levels - 1);
     if (height < m || width < m || height % m || width % m)
         return fail(CTBB_E_ARG, "height and width must be multiples of 2^(levels-1)");
@@ -774,6 +810,8 @@ extern "C" int ctbb_unet_create(int levels, int base_channels, int max_batch, in
         (rc = ualloc_n(u, &u->host_in, P0 * 3, "input staging")) ||
         (rc = ualloc_n(u, &u->wpart, wp_max, "weight-gradient partials")) ||
         (rc = ualloc_n(u, &u->fpart, 1024LL * (base_channels + 1), "final partials")) ||
+        (rc = ualloc_n(u, &u->bpart, ((P0 + kBiasChunk - 1) / kBiasChunk) * (int64_t)ch[L - 1],
+                       "bias partials")) ||
         (rc = ualloc_n(u, &u->lpart, 1024, "loss partials")))
         return bail(rc);
     // wire inputs

@@ -46,8 +46,9 @@ def to_rotating(scan: Scan, vol: np.ndarray) -> np.ndarray:
 
 
 def predict_fixed(net: UNet, vol: np.ndarray) -> np.ndarray:
-    """Network output for every slice of a fixed-frame volume [nz, ny, nx]."""
-    inp = (slice_stacks(vol) / MU_WATER).astype(np.float32)
+    """Network output for every slice of a fixed-frame volume [nz, ny, nx]
+    (inputs clipped to [0, MU_MAX] as in training)."""
+    inp = (slice_stacks(np.clip(vol, 0.0, MU_MAX)) / MU_WATER).astype(np.float32)
     out = np.empty(vol.shape, np.float32)
     for k0 in range(0, vol.shape[0], net.max_batch):
         k1 = min(k0 + net.max_batch, vol.shape[0])
@@ -81,13 +82,25 @@ def generate_pairs(make_scan, seeds, ssm: SSM, n_early: int, n_target: int,
     return pairs
 
 
+MU_MAX = 1.0        # 1/cm; training images are clipped to [0, MU_MAX]
+
+
 def train(net: UNet, pairs: list[Pair], epochs: int, lr: float, seed: int = 0,
-          log=print) -> list[float]:
+          log=print, drop_ends: int = 2) -> list[float]:
     """Adam on the mean squared error (scaled units), slices shuffled; cosine
-    learning-rate decay to lr/20.  Returns the mean loss of every epoch."""
+    learning-rate decay to lr/20.  The `drop_ends` slices at either end of a
+    volume (which absorb attenuation from outside the reconstructed slab) are
+    not trained on, and images are clipped to [0, MU_MAX] so that the few
+    poorly determined voxels of weakly regularised runs do not dominate the
+    loss.  Returns the mean loss of every epoch."""
     rng = np.random.default_rng(seed)
-    inp = np.concatenate([slice_stacks(p.early) for p in pairs]) / MU_WATER
-    tgt = np.concatenate([p.target for p in pairs]) / MU_WATER
+
+    def clip(v):
+        return np.clip(v, 0.0, MU_MAX)
+
+    sl = slice(drop_ends, -drop_ends if drop_ends else None)
+    inp = np.concatenate([slice_stacks(clip(p.early))[sl] for p in pairs]) / MU_WATER
+    tgt = np.concatenate([clip(p.target)[sl] for p in pairs]) / MU_WATER
     inp = inp.astype(np.float32)
     tgt = tgt.astype(np.float32)
     n = inp.shape[0]
@@ -119,17 +132,21 @@ class Run:
 
 
 def run_to_target(r: Recon, target: float, max_iter: int, t0: float, run: Run,
-                  log=print) -> Run:
+                  log=print, cost_fn=None) -> Run:
     """Iterate until the FP64 cost is <= target (checked after every
-    iteration; the cost evaluation is not counted in the time)."""
+    iteration; the cost evaluation is not counted in the time).  cost_fn(r)
+    evaluates the cost (default: the reconstruction's own)."""
+    cost_fn = cost_fn or (lambda rr: rr.cost())
     spent = time.perf_counter() - t0
     while run.iterations < max_iter:
         t = time.perf_counter()
         r.iterate(1)
         spent += time.perf_counter() - t
         run.iterations += 1
-        c = r.cost()
+        c = cost_fn(r)
         run.costs.append(c)
+        log(f"[warmstart] {run.start} iteration {run.iterations}: cost {c:.10e} "
+            f"({spent:.2f} s)")
         if c <= target:
             run.reached = True
             break
@@ -149,7 +166,7 @@ def plain_run(scan: Scan, ssm: SSM, raw, w, target: float, max_iter: int,
 
 
 def warm_run(scan: Scan, ssm: SSM, raw, w, net: UNet, n_early: int, target: float,
-             max_iter: int, log=print) -> Run:
+             max_iter: int, log=print, cost_fn=None) -> Run:
     """n_early plain iterations, the network, the cost check, then ICD to the
     target.  The time includes the early iterations, the rotations, the
     network, the projection of the start image and the two cost evaluations
@@ -171,11 +188,13 @@ def warm_run(scan: Scan, ssm: SSM, raw, w, net: UNet, n_early: int, target: floa
         run.costs = [c_warm if run.accepted else c_plain]
         log(f"[warmstart] start cost plain {c_plain:.8e} warm {c_warm:.8e} -> "
             f"{'accepted' if run.accepted else 'rejected'}")
+        if cost_fn is not None:
+            run.costs = [cost_fn(r)]
         if run.costs[0] <= target:
             run.reached = True
             run.seconds = time.perf_counter() - t0
             return run
-        return run_to_target(r, target, max_iter, t0, run, log)
+        return run_to_target(r, target, max_iter, t0, run, log, cost_fn)
     finally:
         r.free()
 
@@ -184,24 +203,31 @@ def _main():
     import argparse
     import json
     from pathlib import Path
-    from . import configs, init, set_verbose
+    from . import configs, cost as model_cost, init, set_verbose
 
     ap = argparse.ArgumentParser(description="train the warm-start network and "
                                  "measure time to cost tolerance, plain vs warm")
-    ap.add_argument("--config", default="config3")
+    ap.add_argument("--phase", choices=["all", "train", "eval"], default="all")
+    ap.add_argument("--config", default="config3", help="evaluation scan")
+    ap.add_argument("--train-config", default="", help="training scan (default: --config)")
     ap.add_argument("--train", type=int, default=16, help="training phantoms")
     ap.add_argument("--test", type=int, default=3, help="held-out phantoms")
     ap.add_argument("--early", type=int, default=1, help="plain iterations before the network")
     ap.add_argument("--target-its", type=int, default=40, help="iterations of the training targets")
     ap.add_argument("--ref-its", type=int, default=80, help="iterations of the reference runs")
+    ap.add_argument("--dense-from", type=int, default=0,
+                    help="reference runs: cost after every iteration from this one on "
+                         "(every 5th before it)")
     ap.add_argument("--tol", type=float, default=1e-4, help="relative cost tolerance")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--fp16", action="store_true", help="FP16 SSM for every run")
+    ap.add_argument("--fp16-warm", action="store_true",
+                    help="warm runs on the FP16 SSM, judged with the FP32 model's cost")
     ap.add_argument("--workdir", default="/tmp/ctbb_warmstart")
     ap.add_argument("--out", default="", help="JSON summary path")
     ap.add_argument("--params-out", default="", help="trained parameters (.npy)")
+    ap.add_argument("--params-in", default="", help="parameters to evaluate (.npy)")
     a = ap.parse_args()
 
     set_verbose(False)
@@ -209,72 +235,98 @@ def _main():
     wd = Path(a.workdir)
     wd.mkdir(parents=True, exist_ok=True)
 
-    def make_scan(seed):
-        return Scan(configs.write_prm(a.config, wd / f"{a.config}_{seed}.yaml",
-                                      synthetic_seed=seed))
+    def make_scan(cfg, seed):
+        return Scan(configs.write_prm(cfg, wd / f"{cfg}_{seed}.yaml", synthetic_seed=seed))
 
     def log(msg):
         print(msg, flush=True)
 
-    s0 = make_scan(1)
-    g = s0.geom
-    t = time.perf_counter()
-    ssm = SSM.build(g)
-    if a.fp16:
-        ssm.to_fp16()
-    log(f"[warmstart] {a.config}: {g.nx}x{g.ny}x{g.nz}, SSM in {time.perf_counter() - t:.1f} s")
-    pairs = generate_pairs(make_scan, range(1000, 1000 + a.train), ssm, a.early,
-                           a.target_its, log)
-    net = UNet(4, 32, a.batch, g.ny, g.nx, seed=11)
-    t = time.perf_counter()
-    hist = train(net, pairs, a.epochs, a.lr, log=log)
-    t_train = time.perf_counter() - t
-    ms, flops = net.last_timing()
-    log(f"[warmstart] training {t_train:.1f} s; last step {ms:.2f} ms, "
-        f"{flops / (ms * 1e-3) / 1e12:.1f} TFLOP/s in the 3x3 convolutions")
-    if a.params_out:
-        np.save(a.params_out, net.params())
-    results = []
-    for s in range(2000, 2000 + a.test):
-        scan = make_scan(s)
-        raw, w = scan.synthesize(use_gpu=True)
-        # reference: plain ICD for ref-its iterations; target = its final cost
-        ref = Run("reference")
-        r = Recon(ssm, raw, w, scan.slice_index, scan.opts())
-        t0 = time.perf_counter()
-        secs = []
-        for _ in range(a.ref_its):
-            t = time.perf_counter()
-            r.iterate(1)
-            secs.append(time.perf_counter() - t)
-            ref.costs.append(r.cost())
-        r.free()
-        target = ref.costs[-1] * (1 + a.tol)
-        n_plain = next(i + 1 for i, c in enumerate(ref.costs) if c <= target)
-        plain = Run("plain", iterations=n_plain, seconds=float(np.sum(secs[:n_plain])),
-                    reached=True, costs=ref.costs[:n_plain])
-        log(f"[warmstart] seed {s}: converged cost {ref.costs[-1]:.10e}, plain reaches "
-            f"{a.tol:g} after {n_plain} iterations ({plain.seconds:.2f} s)")
-        warm = warm_run(scan, ssm, raw, w, net, a.early, target, a.ref_its, log)
-        results.append(dict(seed=s, converged_cost=ref.costs[-1], target=target,
-                            plain_iterations=plain.iterations, plain_seconds=plain.seconds,
-                            warm_iterations=warm.iterations, warm_seconds=warm.seconds,
-                            warm_reached=warm.reached, warm_accepted=warm.accepted,
-                            plain_costs=plain.costs, warm_costs=warm.costs))
-    summary = dict(config=a.config, fp16=a.fp16, train=a.train, test=a.test, early=a.early,
-                   target_its=a.target_its, ref_its=a.ref_its, tol=a.tol, epochs=a.epochs,
-                   train_seconds=t_train, loss_history=hist,
-                   unet_step_ms=ms, unet_step_tflops=flops / (ms * 1e-3) / 1e12,
-                   results=results)
-    pi = float(np.mean([x["plain_iterations"] for x in results]))
-    wi = float(np.mean([x["warm_iterations"] for x in results]))
-    ps = float(np.mean([x["plain_seconds"] for x in results]))
-    ws = float(np.mean([x["warm_seconds"] for x in results]))
-    summary.update(mean_plain_iterations=pi, mean_warm_iterations=wi,
-                   mean_plain_seconds=ps, mean_warm_seconds=ws,
-                   speedup=ps / ws if ws > 0 else None)
-    log(f"[warmstart] mean: plain {pi:.1f} its / {ps:.2f} s, warm {wi:.1f} its / "
-        f"{ws:.2f} s, speedup {ps / ws:.2f}x")
+    summary = dict(config=a.config, train_config=a.train_config or a.config,
+                   fp16_warm=a.fp16_warm, early=a.early, tol=a.tol)
+    net = None
+    if a.phase in ("all", "train"):
+        tcfg = a.train_config or a.config
+        s0 = make_scan(tcfg, 1)
+        g = s0.geom
+        t = time.perf_counter()
+        ssm = SSM.build(g)
+        log(f"[warmstart] train on {tcfg}: {g.nx}x{g.ny}x{g.nz}, SSM in "
+            f"{time.perf_counter() - t:.1f} s")
+        pairs = generate_pairs(lambda sd: make_scan(tcfg, sd), range(1000, 1000 + a.train),
+                               ssm, a.early, a.target_its, log)
+        ssm.free()
+        net = UNet(4, 32, a.batch, g.ny, g.nx, seed=11)
+        t = time.perf_counter()
+        hist = train(net, pairs, a.epochs, a.lr, log=log)
+        t_train = time.perf_counter() - t
+        ms, flops = net.last_timing()
+        log(f"[warmstart] training {t_train:.1f} s; last step {ms:.2f} ms, "
+            f"{flops / (ms * 1e-3) / 1e12:.1f} TFLOP/s in the 3x3 convolutions")
+        summary.update(train=a.train, target_its=a.target_its, epochs=a.epochs,
+                       train_seconds=t_train, loss_history=hist, unet_step_ms=ms,
+                       unet_step_tflops=flops / (ms * 1e-3) / 1e12)
+        if a.params_out:
+            np.save(a.params_out, net.params())
+        del pairs
+    if a.phase in ("all", "eval"):
+        s0 = make_scan(a.config, 1)
+        g = s0.geom
+        t = time.perf_counter()
+        ssm = SSM.build(g)
+        log(f"[warmstart] evaluate on {a.config}: {g.nx}x{g.ny}x{g.nz}, SSM in "
+            f"{time.perf_counter() - t:.1f} s")
+        ssm_w = ssm
+        if a.fp16_warm:
+            ssm_w = SSM.build(g)
+            ssm_w.to_fp16()
+        if net is None or net.height != g.ny or net.width != g.nx:
+            p = net.params() if net is not None else np.load(a.params_in)
+            net = UNet(4, 32, a.batch, g.ny, g.nx, seed=11)
+            net.set_params(p)
+        results = []
+        for s in range(2000, 2000 + a.test):
+            scan = make_scan(a.config, s)
+            raw, w = scan.synthesize(use_gpu=True)
+            opts = scan.opts()
+            # reference: plain FP32 ICD from zero; target = its final cost
+            r = Recon(ssm, raw, w, scan.slice_index, opts)
+            secs, costs = [], {}
+            for it in range(1, a.ref_its + 1):
+                t = time.perf_counter()
+                r.iterate(1)
+                secs.append(time.perf_counter() - t)
+                if it >= a.dense_from or it % 5 == 0 or it == a.ref_its:
+                    costs[it] = r.cost()
+                    log(f"[warmstart] reference iteration {it}: cost {costs[it]:.10e} "
+                        f"({np.sum(secs):.2f} s)")
+            r.free()
+            final = costs[a.ref_its]
+            target = final * (1 + a.tol)
+            n_plain = min(i for i, c in costs.items() if c <= target)
+            plain = dict(iterations=n_plain, seconds=float(np.sum(secs[:n_plain])))
+            log(f"[warmstart] seed {s}: reference cost {final:.10e}; plain reaches {a.tol:g} "
+                f"after {n_plain} iterations ({plain['seconds']:.2f} s)")
+            cost_fn = None
+            if a.fp16_warm:
+                def cost_fn(rr, raw=raw, w=w, scan=scan, opts=opts):
+                    return model_cost(ssm, raw, w, scan.slice_index, opts, rr.volume())
+            warm = warm_run(scan, ssm_w, raw, w, net, a.early, target, a.ref_its, log, cost_fn)
+            results.append(dict(seed=s, reference_cost=final, target=target,
+                                plain_iterations=n_plain, plain_seconds=plain["seconds"],
+                                plain_seconds_per_iteration=float(np.mean(secs)),
+                                warm_iterations=warm.iterations, warm_seconds=warm.seconds,
+                                warm_reached=warm.reached, warm_accepted=warm.accepted,
+                                reference_costs=costs, warm_costs=warm.costs))
+        pi = float(np.mean([x["plain_iterations"] for x in results]))
+        wi = float(np.mean([x["warm_iterations"] for x in results]))
+        ps = float(np.mean([x["plain_seconds"] for x in results]))
+        wsec = float(np.mean([x["warm_seconds"] for x in results]))
+        summary.update(test=a.test, ref_its=a.ref_its, results=results,
+                       mean_plain_iterations=pi, mean_warm_iterations=wi,
+                       mean_plain_seconds=ps, mean_warm_seconds=wsec,
+                       speedup=ps / wsec if wsec > 0 else None)
+        log(f"[warmstart] mean: plain {pi:.1f} its / {ps:.2f} s, warm {wi:.1f} its / "
+            f"{wsec:.2f} s, speedup {ps / wsec:.2f}x")
     if a.out:
         Path(a.out).write_text(json.dumps(summary, indent=1))
 

@@ -210,6 +210,15 @@ int ctbb_recon_cost(ctbb_recon *r, double *cost);
 /* Copy the running FP32 error sinogram to the host. */
 int ctbb_recon_get_error(const ctbb_recon *r, float *e);
 
+/* FP64 cost of a volume (rotated frame, float32 [nx*ny*nz]) for raw data,
+ * weights (or NULL) and the penalty of opts, without a reconstruction
+ * context: the same definition as ctbb_recon_cost.  The reference has no
+ * global cost; this evaluates the one its updates minimise
+ * (src/penalties.cpp:61-154 with src/icd_iteration.cpp:198-249). */
+int ctbb_cost(const ctbb_ssm *m, const float *raw, const float *weights,
+              const int64_t *slice_index, const ctbb_icd_opts *opts,
+              const float *volume, double *cost);
+
 /* ---- slice rotation -----------------------------------------------------
  * Replaces rotate_slices_rotating2fixed / rotate_slices_fixed2rotating
  * (src/rotate_slices.h:25,29; src/rotate_slices.cpp:48-154 with interp2 of

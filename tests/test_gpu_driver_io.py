@@ -1,0 +1,86 @@
+"""matrix.bin interop (src/generate_system_matrix.cpp:212-303, read at
+src/icd_iteration.cpp:164-179) and the driver end to end (src/main.cpp:89-128,
+output layout of src/setup.cpp:422-438)."""
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu(ct):
+    ct.init(0)
+    return ct
+
+
+def _parse_matrix_bin(path, ncol):
+    """Independent reader of the reference layout: per column (ix fastest)
+    uint64 nnz, then nnz x {int32 index, float32 value}; no header."""
+    buf = open(path, "rb").read()
+    rec = np.dtype([("q", "<i4"), ("v", "<f4")])
+    cp = np.zeros(ncol + 1, np.int64)
+    qs, vs, o = [], [], 0
+    for c in range(ncol):
+        n = int(np.frombuffer(buf, "<u8", 1, o)[0])
+        o += 8
+        r = np.frombuffer(buf, rec, n, o)
+        o += 8 * n
+        qs.append(r["q"])
+        vs.append(r["v"])
+        cp[c + 1] = cp[c] + n
+    assert o == len(buf)
+    return cp, np.concatenate(qs), np.concatenate(vs)
+
+
+def test_matrix_bin_layout_and_round_trip(gpu, make_scan, orc, tmp_path):
+    s = make_scan("tiny_ffs")
+    g = s.geom
+    m = gpu.SSM.build(g)
+    path = tmp_path / "matrix.bin"
+    m.write_file(str(path))
+    cp, q, v = _parse_matrix_bin(path, g.nx * g.ny)
+    ocp, oq, ov = orc.ssm_build(g)
+    assert np.array_equal(cp, ocp) and np.array_equal(q, oq) and np.array_equal(v, ov)
+    m2 = gpu.SSM.read_file(g, str(path))
+    a, b = m.download(), m2.download()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_driver_end_to_end(gpu, ct, tmp_path):
+    from ctbb_icd_amd import configs
+    out = tmp_path / "out"
+    (out / "reconstructions").mkdir(parents=True)
+    prm = configs.write_prm("tiny_ffs", tmp_path / "scan.yaml", output_dir=str(out),
+                            num_iterations=2)
+    driver = ROOT / "ctbb_icd_amd" / "ctbb_icd"
+    runs = []
+    for attempt in range(2):   # second run reuses output_dir/matrix.bin
+        p = subprocess.run([str(driver), str(prm)], capture_output=True, text=True,
+                           timeout=120)
+        print(p.stdout[-600:], flush=True)
+        assert p.returncode == 0, p.stdout + p.stderr
+        runs.append(np.fromfile(out / "reconstruction.img", np.float32))
+    assert "Existing matrix file FOUND" in p.stdout
+    assert (out / "matrix.bin").exists()
+    s = ct.Scan(prm)
+    g = s.geom
+    assert runs[0].size == g.nvox
+    assert np.array_equal(runs[0], runs[1])
+    # the same steps through the library
+    raw, w = s.synthesize(use_gpu=True)
+    m = ct.SSM.build(g)
+    r = ct.Recon(m, raw, w, s.slice_index, s.opts())
+    r.set_volume(np.zeros(g.nvox, np.float32))
+    r.iterate(1)
+    r.iterate(1)
+    x = r.volume()
+    dumped = np.fromfile(out / "reconstructions" / "iteration2.rcn", np.float64)
+    assert np.array_equal(dumped, x.astype(np.float64))
+    assert np.fromfile(out / "reconstructions" / "iteration0.rcn", np.float64).size == g.nvox
+    fixed = ct.rotate_slices(g, s.tube_angles, s.slice_index, x, 1)
+    assert np.array_equal(runs[1], fixed)
