@@ -146,11 +146,12 @@ def main():
     entry_bytes = 6 if args.fp16 else 8
     steps = st.sweep_launches // 3                 # gather, solve, scatter per step
     visits = st.entry_visits / steps               # in-range entries per step
-    # algorithmic bytes per launch (per in-range entry: 4 B error read [+ 4 B
-    # weight read] in the gather, 4 B error write in the scatter; each FOV
-    # column's SSM entries once per sweep, charged to the gather)
-    gather_bytes = visits * (8 if weighted else 4) + st.ssm_entries * entry_bytes * args.steps / steps
-    scatter_bytes = visits * 4
+    # algorithmic bytes per launch (per in-range entry: 4 B error read in the
+    # gather, 4 B error write [+ 4 B weight read] in the scatter -- the
+    # weighted gather reads the kept w * e, the scatter reads w to update it;
+    # each FOV column's SSM entries once per sweep, charged to the gather)
+    gather_bytes = visits * 4 + st.ssm_entries * entry_bytes * args.steps / steps
+    scatter_bytes = visits * (8 if weighted else 4)
     kernels = {
         "icd_gather_kernel": (gather_bytes, st.gather_ms_avg),
         "icd_solve_kernel": (0.0, st.solve_ms_avg),

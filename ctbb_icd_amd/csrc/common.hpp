@@ -116,11 +116,11 @@ struct ctbb_recon {
     int32_t *gb_idx = nullptr;    // [fov entries] ray index (as in the SSM)
     float *gb_val = nullptr;      // [fov entries] FP32 value (FP32 matrices)
     __half *gb_val16 = nullptr;   // [fov entries] FP16 value (FP16 matrices)
-    uint16_t *gb_slot = nullptr;  // [fov entries] column slot in the batch
-    int64_t *gb_ptr = nullptr;    // [nib*ncols_fov+1] start of (b, ib, slot)
+    uint8_t *gb_key = nullptr;    // [fov entries] view in block << 6 | rank % 64
+    uint16_t *gb_perm = nullptr;  // [nib*ncols_fov] column slot of (b, ib, rank)
+    int64_t *gb_ptr = nullptr;    // [chunks+1] start of (b, ib, 64-rank chunk)
+    std::vector<int64_t> h_wbase; // [nbatch+1] first chunk of each batch
     int64_t nib = 0;              // view blocks
-    uint32_t *gb_pairs = nullptr; // per group j: (view block | member << 16)
-    std::vector<int64_t> h_pair_off;    //   in gather dispatch order; [spacing+1]
     std::vector<int64_t> view_stride;   // [spacing] scatter dispatch stride
     float *part1 = nullptr;       // per-step partials [ng][nib][max_batch]
     float *part2 = nullptr;       //   (theta1, theta2)

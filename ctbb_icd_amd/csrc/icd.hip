@@ -291,11 +291,56 @@ __device__ __forceinline__ T run_sum(T v, uint64_t head, int lane)
     return v;
 }
 
+#ifndef CTBB_DPP_SCAN
+#define CTBB_DPP_SCAN 0
+#endif
+
+#if CTBB_DPP_SCAN
+// FP32 run sums with DPP lane shifts instead of LDS permutes.  In-row steps
+// (row_shr 1, 2, 4, 8) treat every 16-lane row start as a run start, so
+// lane l then holds the sum over [max(run start, row start), l]; then the
+// last lane of row 0 / row 2 is added to rows 1 / 3 (row_bcast:15), and lane
+// 31 to rows 2 and 3 (row_bcast:31), where the run reaches back that far.
+// Fixed order of additions: deterministic.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_shift(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF,
+                                                      true));
+}
+
+template <>
+__device__ __forceinline__ float run_sum<float>(float v, uint64_t head, int lane)
+{
+    const uint64_t below = ~0ull >> (63 - lane);
+    const int start = 63 - __clzll((unsigned long long)(head & below));
+    const int rstart = max(start, lane & ~15);
+    float t = dpp_shift<0x111, 0xF>(v);
+    if (lane - 1 >= rstart) v += t;
+    t = dpp_shift<0x112, 0xF>(v);
+    if (lane - 2 >= rstart) v += t;
+    t = dpp_shift<0x114, 0xF>(v);
+    if (lane - 4 >= rstart) v += t;
+    t = dpp_shift<0x118, 0xF>(v);
+    if (lane - 8 >= rstart) v += t;
+    t = dpp_shift<0x142, 0xA>(v);
+    if ((lane & 16) && start < (lane & ~15)) v += t;
+    t = dpp_shift<0x143, 0xC>(v);
+    if (lane >= 32 && start < 32) v += t;
+    return v;
+}
+#endif
+
 // Head mask of a wavefront whose lanes hold sorted keys (invalid lanes, at
 // the tail, carry key -1) and run-end test for lane `lane`.
 __device__ __forceinline__ uint64_t run_heads(int key, int lane)
 {
+#if CTBB_DPP_SCAN
+    // wave_shr:1 -- lane l reads lane l - 1 (lane 0 keeps the old value)
+    const int prev = __builtin_amdgcn_update_dpp(key, key, 0x138, 0xF, 0xF, false);
+#else
     const int prev = __shfl_up(key, 1, 64);
+#endif
     return __ballot(lane == 0 || prev != key);
 }
 
@@ -313,23 +358,26 @@ struct StepArgs {
 // ---------------------------------------------------------------------------
 // Phase A1 of a step: view-blocked gather.
 //
-// The SSM views are cut into blocks of kViewBlock views; for every batch the
-// matrix entries of its columns are copied in (view block ib, column slot)
-// order, so the entries of one block and a run of slots are contiguous.  A
-// workgroup takes one (view block, group member) pair and kSweepBlock slots
-// (wavefront w: slots [64w, 64w + 64) of the chunk).  Lanes hold consecutive
-// entries and form a * (w e) at sample idx + slice offset (and, on the first
-// sweep, w a^2); runs of equal slots are summed with shuffles and the last
-// lane of a run adds into the slot's LDS accumulator (one wavefront per
-// slot, in entry order: deterministic).  The per-(member, block, slot) sums
-// go to a dense partials array; phase A2 adds them over the blocks in a
-// fixed order.  The pairs are dispatched in order of the first data view
-// they touch (ib * kViewBlock + c_k - n_sm/2), so workgroups resident at the
-// same time read the same band of a few data views of the error sinogram --
-// a band that fits in L2 -- whichever slice they work on; logical workgroups
-// go to the XCDs in contiguous ranges.  FP16 matrices: values are summed
-// unscaled and the per-column scale is applied in phase A2 (it factors out
-// of both sums).
+// The SSM views are cut into blocks of kViewBlock views.  For every batch
+// and block, the matrix entries of the batch's columns are copied with the
+// columns ranked by where their footprint starts on the detector in that
+// block (channel, then row), in chunks of 64 ranks; inside a chunk the
+// entries are ordered (view in block, rank).  A workgroup takes one view
+// block and 4 chunks (one per wavefront) and loops over the members of the
+// slice group: the chunk's entries are read once per member from L1/L2, and
+// neighbouring lanes read neighbouring samples of the error sinogram, so a
+// 64-lane gather touches few cache lines.  Lanes form a * (w e) at sample
+// idx + slice offset (and, on the first sweep, w a^2); runs of equal entry
+// keys (view in block, rank) are summed with shuffles and the run's last
+// lane adds into the wavefront's LDS accumulator of that key (one wavefront
+// per chunk, in entry order: deterministic).  Per member, each lane adds its
+// rank's kViewBlock accumulators in view order and writes the partial of
+// (member, block, slot); phase A2 adds the partials over the blocks in a
+// fixed order.  Workgroups are dispatched in block order, so the chunks of
+// one block -- which share the error-sinogram lines of a member's band --
+// run together on one XCD (logical workgroups go to the XCDs in contiguous
+// ranges).  FP16 matrices: values are summed unscaled and the per-column
+// scale is applied in phase A2 (it factors out of both sums).
 #ifndef CTBB_VIEW_BLOCK
 #define CTBB_VIEW_BLOCK 4
 #endif
@@ -343,20 +391,22 @@ struct GatherArgs {
     const int32_t *idx;       // view-blocked copy (see above)
     const float *val;
     const __half *val16;
-    const uint16_t *slot;     // column slot of each entry
-    const int64_t *ptr;       // global (b, ib, slot) starts
+    const uint8_t *key;       // view in block << 6 | rank % 64
+    const uint16_t *perm;     // column slot of each (b, ib, rank)
+    const int64_t *ptr;       // starts of the (b, ib, chunk) entry ranges
     int64_t base;             // nib * (first FOV index of the batch)
+    int64_t wbase;            // first chunk of the batch
+    int64_t nw;               // 64-rank chunks of the batch
     float *part1, *part2;     // [ng][nib][nc]
     int64_t nib;
     const int64_t *si_half;   // [nz] central view - n_sm/2
-    const uint32_t *pairs;    // (ib | member << 16) of the step's group, in
-    int64_t npairs;           //   dispatch order
-    int64_t nsc;              // slot chunks
+    int64_t nsc;              // workgroups per view block
     int64_t nsm, readings;
 };
 
-// View block ib of slice half-offset h holds data views [8 ib + h,
-// min(8 ib + 8, n_sm) + h); true if any of them is inside [0, readings).
+// View block ib of slice half-offset h holds data views [kViewBlock ib + h,
+// min(kViewBlock (ib + 1), n_sm) + h); true if any of them is inside
+// [0, readings).
 __host__ __device__ inline bool block_valid(int64_t ib, int64_t h, int64_t nsm,
                                             int64_t readings)
 {
@@ -376,76 +426,91 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                                                                  StepArgs sa,
                                                                  GatherArgs g)
 {
-    __shared__ float acc1[kSweepBlock], acc2[kSweepBlock];
+    constexpr int kAcc = 64 * kViewBlock;
+    __shared__ float acc1[kWaves * kAcc], acc2[kWaves * kAcc];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t nwg = g.npairs * g.nsc;
+    const int64_t nwg = g.nib * g.nsc;
     const int64_t per_xcd = (nwg + kXcd - 1) / kXcd;
     const int64_t L = (int64_t)(blockIdx.x % kXcd) * per_xcd + blockIdx.x / kXcd;
     if (L >= nwg)
         return;
-    const int64_t pi = L / g.nsc, sc = L - pi * g.nsc;
-    const uint32_t pr = g.pairs[pi];
-    const int64_t ib = pr & 0xffffu, v = pr >> 16;
-    const int64_t sw0 = sc * kSweepBlock + wave * 64;
-    if (sw0 >= sa.nc)
+    const int64_t ib = L / g.nsc;
+    const int64_t w = (L - ib * g.nsc) * kWaves + wave;
+    if (w >= g.nw)
         return;
-    const int64_t sw1 = min(sw0 + 64, sa.nc);
-    const int64_t *ptr = g.ptr + g.base + ib * sa.nc;
-    const int64_t lo = ptr[sw0], hi = ptr[sw1];
-    float *a1 = acc1 + wave * 64, *a2 = acc2 + wave * 64;
+    const int64_t *ptr = g.ptr + g.wbase + ib * g.nw;
+    const int64_t lo = ptr[w], hi = ptr[w + 1];
+    const int64_t rk = w * 64 + lane;
+    const bool own = rk < sa.nc;
+    const int64_t slot = own ? (int64_t)g.perm[g.base + ib * sa.nc + rk] : 0;
+    float *a1 = acc1 + wave * kAcc, *a2 = acc2 + wave * kAcc;
     const float *__restrict__ ew = a.ew;
-    const float *__restrict__ w = a.w;
+    const float *__restrict__ wt = a.w;
     const int64_t ds = a.data_size;
-    const int64_t k = sa.j + a.spacing * v;
-    const int64_t off = a.slice_off[k];
-    a1[lane] = 0.0f;
-    if (THETA2)
-        a2[lane] = 0.0f;
-    for (int64_t p0 = lo; p0 < hi; p0 += 64 * kGatherUnroll) {
-        int32_t iv[kGatherUnroll], sl[kGatherUnroll];
-        float av[kGatherUnroll];
-        bool ok[kGatherUnroll];
+    for (int64_t v = 0; v < sa.ng; v++) {
+        const int64_t k = sa.j + a.spacing * v;
+        if (!block_valid(ib, g.si_half[k], g.nsm, g.readings))
+            continue;
+        const int64_t off = a.slice_off[k];
 #pragma unroll
-        for (int u = 0; u < kGatherUnroll; u++) {
-            const int64_t p = p0 + u * 64 + lane;
-            ok[u] = p < hi;
-            iv[u] = ok[u] ? g.idx[p] : 0;
-            av[u] = ok[u] ? gb_value<FP16>(g, p) : 0.0f;
-            sl[u] = ok[u] ? (int32_t)g.slot[p] : -1;
+        for (int q = 0; q < kViewBlock; q++) {
+            a1[q * 64 + lane] = 0.0f;
+            if (THETA2)
+                a2[q * 64 + lane] = 0.0f;
         }
-        int64_t r[kGatherUnroll];
+        for (int64_t p0 = lo; p0 < hi; p0 += 64 * kGatherUnroll) {
+            int32_t iv[kGatherUnroll], kv[kGatherUnroll];
+            float av[kGatherUnroll];
+            bool ok[kGatherUnroll];
 #pragma unroll
-        for (int u = 0; u < kGatherUnroll; u++) {
-            r[u] = (int64_t)iv[u] + off;
-            const bool in = ok[u] && r[u] >= 0 && r[u] < ds;
-            av[u] = in ? av[u] : 0.0f;
-            r[u] = in ? r[u] : 0;
-        }
-        float ev[kGatherUnroll], wv[kGatherUnroll];
+            for (int u = 0; u < kGatherUnroll; u++) {
+                const int64_t p = p0 + u * 64 + lane;
+                ok[u] = p < hi;
+                iv[u] = ok[u] ? g.idx[p] : 0;
+                av[u] = ok[u] ? gb_value<FP16>(g, p) : 0.0f;
+                kv[u] = ok[u] ? (int32_t)g.key[p] : -1;
+            }
+            int64_t r[kGatherUnroll];
 #pragma unroll
-        for (int u = 0; u < kGatherUnroll; u++) {
-            ev[u] = ew[r[u]];
-            wv[u] = (THETA2 && w) ? w[r[u]] : 1.0f;
-        }
+            for (int u = 0; u < kGatherUnroll; u++) {
+                r[u] = (int64_t)iv[u] + off;
+                const bool in = ok[u] && r[u] >= 0 && r[u] < ds;
+                av[u] = in ? av[u] : 0.0f;
+                r[u] = in ? r[u] : 0;
+            }
+            float ev[kGatherUnroll], wv[kGatherUnroll];
 #pragma unroll
-        for (int u = 0; u < kGatherUnroll; u++) {
-            if (p0 + u * 64 >= hi)
-                break;
-            const uint64_t head = run_heads(sl[u], lane);
-            const float t1 = run_sum(av[u] * ev[u], head, lane);
-            const float t2 = THETA2 ? run_sum((wv[u] * av[u]) * av[u], head, lane) : 0.0f;
-            if (ok[u] && run_end(head, lane)) {
-                a1[sl[u] - sw0] += t1;
-                if (THETA2)
-                    a2[sl[u] - sw0] += t2;
+            for (int u = 0; u < kGatherUnroll; u++) {
+                ev[u] = ew[r[u]];
+                wv[u] = (THETA2 && wt) ? wt[r[u]] : 1.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < kGatherUnroll; u++) {
+                if (p0 + u * 64 >= hi)
+                    break;
+                const uint64_t head = run_heads(kv[u], lane);
+                const float t1 = run_sum(av[u] * ev[u], head, lane);
+                const float t2 = THETA2 ? run_sum((wv[u] * av[u]) * av[u], head, lane) : 0.0f;
+                if (ok[u] && run_end(head, lane)) {
+                    a1[kv[u]] += t1;
+                    if (THETA2)
+                        a2[kv[u]] += t2;
+                }
             }
         }
-    }
-    if (sw0 + lane < sw1) {
-        const int64_t o = (v * g.nib + ib) * sa.nc + sw0 + lane;
-        g.part1[o] = a1[lane];
-        if (THETA2)
-            g.part2[o] = a2[lane];
+        float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kViewBlock; q++) {
+            s1 += a1[q * 64 + lane];
+            if (THETA2)
+                s2 += a2[q * 64 + lane];
+        }
+        if (own) {
+            const int64_t o = (v * g.nib + ib) * sa.nc + slot;
+            g.part1[o] = s1;
+            if (THETA2)
+                g.part2[o] = s2;
+        }
     }
 }
 
@@ -1440,25 +1505,30 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
     if (r->gb_idx) (void)hipFree(r->gb_idx);
     if (r->gb_val) (void)hipFree(r->gb_val);
     if (r->gb_val16) (void)hipFree(r->gb_val16);
-    if (r->gb_slot) (void)hipFree(r->gb_slot);
+    if (r->gb_key) (void)hipFree(r->gb_key);
+    if (r->gb_perm) (void)hipFree(r->gb_perm);
     if (r->gb_ptr) (void)hipFree(r->gb_ptr);
-    if (r->gb_pairs) (void)hipFree(r->gb_pairs);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return CTBB_OK;
 }
 
-// Transposed copy of every batch's SSM entries for the sample-driven
-// scatter: entries grouped by batch, then by (SSM view, channel) segment,
-// sorted by (row, column slot) inside a segment.
-// View-blocked batch copy (gather): entry counts and copy of every (FOV
-// column, view block); fov_batch/fov_slot/fov_base give the batch, slot and
-// nib * (first FOV index of the batch) of each FOV column index.
-__global__ void gb_count_kernel(const int64_t *col_ptr, const int32_t *view_off,
-                                const int32_t *cols, int64_t ncols,
-                                const int64_t *fov_base, const int32_t *fov_nc,
-                                const int32_t *fov_slot, int64_t nsm, int64_t nib,
-                                uint32_t *cnt)
+// View-blocked batch copy (gather).  Per batch b and view block ib the
+// batch's column slots are put in the order of where their footprint starts
+// on the detector in the block's first view (channel, then row), so that
+// consecutive ranks read neighbouring samples of the error sinogram.  Ranks
+// are cut into wavefront chunks of 64; a chunk's entries are stored as
+// (view in block q, rank, SSM order), each with the byte q << 6 | rank % 64.
+//
+// gb_key_kernel: sort key of every (FOV column, view block) -- the first
+// entry's ray index within its view (row + nr * ch; 2^17 when the column has
+// no entry in the block) << 14 | slot (unique keys: the order is fixed).
+__global__ void gb_key_kernel(const int64_t *col_ptr, const int32_t *idx,
+                              const int32_t *view_off, const int32_t *cols,
+                              int64_t ncols, const int64_t *fov_base,
+                              const int32_t *fov_nc, const int32_t *fov_slot,
+                              int64_t nsm, int64_t nib, int64_t plane,
+                              uint32_t *keys)
 {
     const int64_t n = ncols * nib;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
@@ -1469,35 +1539,114 @@ __global__ void gb_count_kernel(const int64_t *col_ptr, const int32_t *view_off,
         const int64_t v0 = ib * kViewBlock, v1 = min(v0 + kViewBlock, nsm);
         const int64_t e0 = vo[v0];
         const int64_t e1 = v1 < nsm ? (int64_t)vo[v1] : col_ptr[col + 1] - col_ptr[col];
-        cnt[fov_base[i] + ib * fov_nc[i] + fov_slot[i]] = (uint32_t)(e1 - e0);
+        const uint32_t pos =
+            e1 > e0 ? (uint32_t)(idx[col_ptr[col] + e0] % plane) : (1u << 17);
+        keys[fov_base[i] + ib * fov_nc[i] + fov_slot[i]] = pos << 14 | (uint32_t)fov_slot[i];
     }
 }
 
-__global__ void gb_fill_kernel(const int64_t *col_ptr, const int32_t *idx,
-                               const float *val, const __half *val16,
-                               const int32_t *view_off, const int32_t *cols,
-                               int64_t ncols, const int64_t *fov_base,
-                               const int32_t *fov_nc, const int32_t *fov_slot,
-                               int64_t nsm, int64_t nib, const int64_t *gptr,
-                               int32_t *gidx, float *gval, __half *gval16,
-                               uint16_t *gslot)
+// One workgroup per (batch, view block): bitonic sort of the batch's keys in
+// LDS (dynamic, next power of two >= the largest batch), then rank -> slot.
+__global__ __launch_bounds__(1024) void gb_sort_kernel(const uint32_t *keys,
+                                                       const int64_t *bptr,
+                                                       int64_t nib, uint16_t *perm)
 {
-    const int64_t n = ncols * nib;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = t / nib, ib = t - i * nib;
-        const int64_t col = cols[i];
-        const int64_t src = col_ptr[col] + view_off[col * nsm + ib * kViewBlock];
-        const int64_t o = fov_base[i] + ib * fov_nc[i] + fov_slot[i];
-        const int64_t dst = gptr[o], len = gptr[o + 1] - dst;
-        for (int64_t q = 0; q < len; q++) {
-            gidx[dst + q] = idx[src + q];
-            if (gval16)
-                gval16[dst + q] = val16[src + q];
-            else
-                gval[dst + q] = val[src + q];
-            gslot[dst + q] = (uint16_t)fov_slot[i];
+    extern __shared__ uint32_t sk[];
+    const int64_t b = blockIdx.x / nib, ib = blockIdx.x - b * nib;
+    const int64_t c0 = bptr[b], nc = bptr[b + 1] - c0;
+    const int64_t o = nib * c0 + ib * nc;
+    int P = 1;
+    while (P < nc)
+        P <<= 1;
+    for (int i = threadIdx.x; i < P; i += blockDim.x)
+        sk[i] = i < nc ? keys[o + i] : 0xffffffffu;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            for (int i = threadIdx.x; i < P; i += blockDim.x) {
+                const int l = i ^ jj;
+                if (l > i) {
+                    const uint32_t x = sk[i], y = sk[l];
+                    if ((x > y) == ((i & k) == 0)) {
+                        sk[i] = y;
+                        sk[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
         }
+    for (int i = threadIdx.x; i < nc; i += blockDim.x)
+        perm[o + i] = (uint16_t)(sk[i] & 0x3fffu);
+}
+
+// Entries of column col in SSM view v (0 past the last view).
+__device__ __forceinline__ int64_t gb_view_count(const int64_t *col_ptr,
+                                                 const int32_t *view_off, int64_t col,
+                                                 int64_t nsm, int64_t v)
+{
+    if (v >= nsm)
+        return 0;
+    const int32_t *vo = view_off + col * nsm;
+    const int64_t e1 = v + 1 < nsm ? (int64_t)vo[v + 1] : col_ptr[col + 1] - col_ptr[col];
+    return e1 - vo[v];
+}
+
+// Inclusive sum over the lanes of a wavefront.
+__device__ __forceinline__ int64_t wave_incl_sum(int64_t x, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int64_t u = __shfl_up(x, off, 64);
+        if (lane >= off)
+            x += u;
+    }
+    return x;
+}
+
+// One workgroup per (batch, view block), wavefront w takes chunks w, w+4, ..:
+// FILL = false writes the entry count of every chunk, FILL = true copies the
+// chunk's entries to the starts the scan of the counts gave.
+template <bool FILL>
+__global__ __launch_bounds__(256) void gb_chunk_kernel(
+    const int64_t *col_ptr, const int32_t *idx, const float *val,
+    const __half *val16, const int32_t *view_off, const int32_t *cols,
+    const int64_t *bptr, const int64_t *wbase, const uint16_t *perm, int64_t nsm,
+    int64_t nib, uint32_t *wcnt, const int64_t *gptr, int32_t *gidx, float *gval,
+    __half *gval16, uint8_t *gkey)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t b = blockIdx.x / nib, ib = blockIdx.x - b * nib;
+    const int64_t c0 = bptr[b], nc = bptr[b + 1] - c0;
+    const int64_t nw = (nc + 63) / 64;
+    const int64_t o = nib * c0 + ib * nc;
+    for (int64_t w = wave; w < nw; w += 4) {
+        const int64_t rk = w * 64 + lane;
+        const bool ok = rk < nc;
+        const int64_t col = ok ? cols[c0 + perm[o + rk]] : 0;
+        const int64_t ci = wbase[b] + ib * nw + w;
+        int64_t seg = FILL ? gptr[ci] : 0;
+        for (int q = 0; q < kViewBlock; q++) {
+            const int64_t v = ib * kViewBlock + q;
+            const int64_t n = ok ? gb_view_count(col_ptr, view_off, col, nsm, v) : 0;
+            const int64_t inc = wave_incl_sum(n, lane);
+            const int64_t tot = __shfl(inc, 63, 64);
+            if (FILL && n > 0) {
+                const int64_t src = col_ptr[col] + view_off[col * nsm + v];
+                const int64_t dst = seg + inc - n;
+                const uint8_t kb = (uint8_t)(q << 6 | lane);
+                for (int64_t e = 0; e < n; e++) {
+                    gidx[dst + e] = idx[src + e];
+                    if (gval16)
+                        gval16[dst + e] = val16[src + e];
+                    else
+                        gval[dst + e] = val[src + e];
+                    gkey[dst + e] = kb;
+                }
+            }
+            seg += tot;
+        }
+        if (!FILL && lane == 0)
+            wcnt[ci] = (uint32_t)seg;
     }
 }
 
@@ -1508,36 +1657,55 @@ static int build_gather_blocks(ctbb_recon *r, const std::vector<int64_t> &bptr)
     const int64_t nsm = g->n_views_sm, nib = (nsm + kViewBlock - 1) / kViewBlock;
     const int64_t ncols = r->ncols_fov, n = nib * ncols;
     const bool fp16 = m->precision == CTBB_SSM_FP16;
+    static_assert(kViewBlock <= 4, "the entry byte holds the view in block in 2 bits");
     r->nib = nib;
     std::vector<int64_t> base(ncols);
     std::vector<int32_t> nc(ncols), slot(ncols);
-    for (int64_t b = 0; b < r->nbatch; b++)
+    r->h_wbase.assign(r->nbatch + 1, 0);
+    int64_t max_nc = 1;
+    for (int64_t b = 0; b < r->nbatch; b++) {
+        const int64_t nb = bptr[b + 1] - bptr[b];
+        max_nc = std::max(max_nc, nb);
+        r->h_wbase[b + 1] = r->h_wbase[b] + nib * ((nb + 63) / 64);
         for (int64_t i = bptr[b]; i < bptr[b + 1]; i++) {
             base[i] = nib * bptr[b];
-            nc[i] = (int32_t)(bptr[b + 1] - bptr[b]);
+            nc[i] = (int32_t)nb;
             slot[i] = (int32_t)(i - bptr[b]);
         }
-    int64_t *d_base = nullptr, *sums = nullptr;
+    }
+    int P = 1;
+    while (P < max_nc)
+        P <<= 1;
+    if (P > (1 << 14))
+        return fail(CTBB_E_ARG, "batches of more than 16384 columns");
+    const int64_t nchunks = r->h_wbase[r->nbatch];
+    const int64_t nblk = r->nbatch * nib;
+    int64_t *d_base = nullptr, *sums = nullptr, *d_wbase = nullptr;
     int32_t *d_nc = nullptr, *d_slot = nullptr;
-    uint32_t *cnt = nullptr;
-    const int64_t nchunk = (n + kScanChunk - 1) / kScanChunk;
+    uint32_t *keys = nullptr, *wcnt = nullptr;
+    const int64_t nscan = (nchunks + kScanChunk - 1) / kScanChunk;
     int rc = CTBB_OK;
     auto cleanup = [&]() {
         if (d_base) (void)hipFree(d_base);
         if (sums) (void)hipFree(sums);
+        if (d_wbase) (void)hipFree(d_wbase);
         if (d_nc) (void)hipFree(d_nc);
         if (d_slot) (void)hipFree(d_slot);
-        if (cnt) (void)hipFree(cnt);
+        if (keys) (void)hipFree(keys);
+        if (wcnt) (void)hipFree(wcnt);
     };
     const int64_t ne = std::max<int64_t>(1, r->ssm_fov_entries);
     if ((rc = dev_alloc_n(&d_base, std::max<int64_t>(1, ncols), "FOV batch bases")) ||
         (rc = dev_alloc_n(&d_nc, std::max<int64_t>(1, ncols), "FOV batch sizes")) ||
         (rc = dev_alloc_n(&d_slot, std::max<int64_t>(1, ncols), "FOV slots")) ||
-        (rc = dev_alloc_n(&cnt, std::max<int64_t>(1, n), "block counts")) ||
-        (rc = dev_alloc_n(&sums, std::max<int64_t>(1, nchunk), "chunk sums")) ||
-        (rc = dev_alloc_n(&r->gb_ptr, n + 1, "block pointers")) ||
+        (rc = dev_alloc_n(&d_wbase, r->nbatch + 1, "chunk bases")) ||
+        (rc = dev_alloc_n(&keys, std::max<int64_t>(1, n), "rank keys")) ||
+        (rc = dev_alloc_n(&wcnt, std::max<int64_t>(1, nchunks), "chunk counts")) ||
+        (rc = dev_alloc_n(&sums, std::max<int64_t>(1, nscan), "chunk sums")) ||
+        (rc = dev_alloc_n(&r->gb_perm, std::max<int64_t>(1, n), "rank slots")) ||
+        (rc = dev_alloc_n(&r->gb_ptr, nchunks + 1, "chunk pointers")) ||
         (rc = dev_alloc_n(&r->gb_idx, ne, "blocked indices")) ||
-        (rc = dev_alloc_n(&r->gb_slot, ne, "blocked slots")) ||
+        (rc = dev_alloc_n(&r->gb_key, ne, "blocked keys")) ||
         (rc = fp16 ? dev_alloc_n(&r->gb_val16, ne, "blocked values")
                    : dev_alloc_n(&r->gb_val, ne, "blocked values"))) {
         cleanup();
@@ -1549,38 +1717,47 @@ static int build_gather_blocks(ctbb_recon *r, const std::vector<int64_t> &bptr)
         if (e == hipSuccess) e = hipMemcpy(d_nc, nc.data(), ncols * 4, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(d_slot, slot.data(), ncols * 4, hipMemcpyHostToDevice);
     }
+    if (e == hipSuccess)
+        e = hipMemcpy(d_wbase, r->h_wbase.data(), (r->nbatch + 1) * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess && n) {
-        hipLaunchKernelGGL(gb_count_kernel, dim3(4096), dim3(256), 0, 0, m->col_ptr,
-                           m->view_off, r->cols, ncols, d_base, d_nc, d_slot, nsm, nib, cnt);
-        hipLaunchKernelGGL(chunk_sum_kernel, dim3((unsigned)nchunk), dim3(256), 0, 0, cnt,
-                           n, sums);
+        hipLaunchKernelGGL(gb_key_kernel, dim3(4096), dim3(256), 0, 0, m->col_ptr, m->idx,
+                           m->view_off, r->cols, ncols, d_base, d_nc, d_slot, nsm, nib,
+                           g->n_rows * g->n_channels, keys);
+        hipLaunchKernelGGL(gb_sort_kernel, dim3((unsigned)nblk), dim3(1024),
+                           (size_t)P * 4, 0, keys, r->batch_ptr, nib, r->gb_perm);
+        hipLaunchKernelGGL(gb_chunk_kernel<false>, dim3((unsigned)nblk), dim3(256), 0, 0,
+                           m->col_ptr, m->idx, m->val, m->val16, m->view_off, r->cols,
+                           r->batch_ptr, d_wbase, r->gb_perm, nsm, nib, wcnt, nullptr,
+                           nullptr, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL(chunk_sum_kernel, dim3((unsigned)nscan), dim3(256), 0, 0, wcnt,
+                           nchunks, sums);
         e = hipGetLastError();
     }
-    std::vector<int64_t> hs(nchunk);
-    if (e == hipSuccess && nchunk)
-        e = hipMemcpy(hs.data(), sums, nchunk * 8, hipMemcpyDeviceToHost);
+    std::vector<int64_t> hs(nscan);
+    if (e == hipSuccess && nscan)
+        e = hipMemcpy(hs.data(), sums, nscan * 8, hipMemcpyDeviceToHost);
     int64_t run = 0;
-    for (int64_t i = 0; i < nchunk; i++) {
+    for (int64_t i = 0; i < nscan; i++) {
         const int64_t v = hs[i];
         hs[i] = run;
         run += v;
     }
-    if (e == hipSuccess && nchunk)
-        e = hipMemcpy(sums, hs.data(), nchunk * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess && n) {
-        hipLaunchKernelGGL(chunk_scan_kernel, dim3((unsigned)nchunk), dim3(256), 0, 0, cnt,
-                           n, sums, r->gb_ptr);
+    if (e == hipSuccess && nscan)
+        e = hipMemcpy(sums, hs.data(), nscan * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nscan) {
+        hipLaunchKernelGGL(chunk_scan_kernel, dim3((unsigned)nscan), dim3(256), 0, 0, wcnt,
+                           nchunks, sums, r->gb_ptr);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
-        e = hipMemcpy(r->gb_ptr + n, &run, 8, hipMemcpyHostToDevice);
+        e = hipMemcpy(r->gb_ptr + nchunks, &run, 8, hipMemcpyHostToDevice);
     if (e == hipSuccess && run != r->ssm_fov_entries)
         e = hipErrorUnknown;
     if (e == hipSuccess && n) {
-        hipLaunchKernelGGL(gb_fill_kernel, dim3(8192), dim3(256), 0, 0, m->col_ptr, m->idx,
-                           m->val, m->val16, m->view_off, r->cols, ncols, d_base, d_nc,
-                           d_slot, nsm, nib, r->gb_ptr, r->gb_idx, r->gb_val, r->gb_val16,
-                           r->gb_slot);
+        hipLaunchKernelGGL(gb_chunk_kernel<true>, dim3((unsigned)nblk), dim3(256), 0, 0,
+                           m->col_ptr, m->idx, m->val, m->val16, m->view_off, r->cols,
+                           r->batch_ptr, d_wbase, r->gb_perm, nsm, nib, nullptr, r->gb_ptr,
+                           r->gb_idx, r->gb_val, r->gb_val16, r->gb_key);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
@@ -1591,6 +1768,9 @@ static int build_gather_blocks(ctbb_recon *r, const std::vector<int64_t> &bptr)
     return CTBB_OK;
 }
 
+// Transposed copy of every batch's SSM entries for the sample-driven
+// scatter: entries grouped by batch, then by (SSM view, channel) segment,
+// sorted by (row, column slot) inside a segment.
 static int build_transposed_batches(ctbb_recon *r, const std::vector<int32_t> &cols,
                                     const std::vector<int64_t> &bptr)
 {
@@ -1845,9 +2025,7 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
         return bail(rc);
     if ((rc = build_gather_blocks(r, bptr)) != CTBB_OK)
         return bail(rc);
-    // partials and theta2 cache; per slice group, the gather's (view block,
-    // member) pairs ordered by the first data view they touch, so that
-    // consecutive workgroups read the same band of the error sinogram
+    // partials and theta2 cache
     {
         const int64_t ng_max = (g->nz + r->spacing - 1) / r->spacing;
         const int64_t np = std::max<int64_t>(1, ng_max * r->nib * max_batch);
@@ -1856,8 +2034,6 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
             (rc = dev_alloc_n(&r->th2, r->nvox, "theta2")))
             return bail(rc);
         r->th2_valid = false;
-        if (r->nib > 0xffff || ng_max > 0xffff)
-            return bail(fail(CTBB_E_UNSUPPORTED, "too many view blocks or slices per group"));
         // scatter dispatch stride: median view distance of consecutive members
         r->view_stride.assign(std::max<int64_t>(1, r->spacing), 1);
         for (int64_t j = 0; j < r->spacing && j < g->nz; j++) {
@@ -1868,33 +2044,6 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
                 continue;
             std::nth_element(d.begin(), d.begin() + d.size() / 2, d.end());
             r->view_stride[j] = std::max<int64_t>(1, d[d.size() / 2]);
-        }
-        std::vector<uint32_t> pairs;
-        r->h_pair_off.assign(1, 0);
-        for (int64_t j = 0; j < r->spacing; j++) {
-            std::vector<std::pair<int64_t, uint32_t>> pj;
-            for (int64_t k = j, v = 0; k < g->nz; k += r->spacing, v++)
-                for (int64_t ib = 0; ib < r->nib; ib++)
-                    if (block_valid(ib, r->h_si_half[k], g->n_views_sm, g->readings))
-                        pj.emplace_back(ib * kViewBlock + r->h_si_half[k],
-                                        (uint32_t)ib | ((uint32_t)v << 16));
-            std::stable_sort(pj.begin(), pj.end(),
-                             [](const std::pair<int64_t, uint32_t> &x,
-                                const std::pair<int64_t, uint32_t> &y) {
-                                 return x.first < y.first;
-                             });
-            for (auto &q : pj)
-                pairs.push_back(q.second);
-            r->h_pair_off.push_back((int64_t)pairs.size());
-        }
-        if ((rc = dev_alloc_n(&r->gb_pairs, std::max<size_t>(1, pairs.size()), "gather pairs")))
-            return bail(rc);
-        if (!pairs.empty()) {
-            hipError_t he2 = hipMemcpy(r->gb_pairs, pairs.data(), pairs.size() * 4,
-                                       hipMemcpyHostToDevice);
-            if (he2 != hipSuccess)
-                return bail(fail(CTBB_E_HIP, std::string("gather pairs: ") +
-                                                 hipGetErrorString(he2)));
         }
     }
     *out = r;
@@ -2059,7 +2208,8 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
     ga.idx = r->gb_idx;
     ga.val = r->gb_val;
     ga.val16 = r->gb_val16;
-    ga.slot = r->gb_slot;
+    ga.key = r->gb_key;
+    ga.perm = r->gb_perm;
     ga.ptr = r->gb_ptr;
     ga.part1 = r->part1;
     ga.part2 = r->part2;
@@ -2117,10 +2267,10 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                 sa.j = j;
                 sa.ng = (r->g.nz - j + r->spacing - 1) / r->spacing;
                 ga.base = r->nib * c0;
-                ga.pairs = r->gb_pairs + r->h_pair_off[j];
-                ga.npairs = r->h_pair_off[j + 1] - r->h_pair_off[j];
-                ga.nsc = (nc + kSweepBlock - 1) / kSweepBlock;
-                const int64_t nwg = ga.npairs * ga.nsc;
+                ga.wbase = r->h_wbase[b];
+                ga.nw = (nc + 63) / 64;
+                ga.nsc = (ga.nw + kWaves - 1) / kWaves;
+                const int64_t nwg = r->nib * ga.nsc;
                 const int64_t ntask = kXcd * ((nwg + kXcd - 1) / kXcd);
                 int64_t vlo = INT64_MAX, vhi = INT64_MIN;
                 for (int64_t k = j; k < r->g.nz; k += r->spacing) {
