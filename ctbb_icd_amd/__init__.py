@@ -68,7 +68,7 @@ class IcdOpts(C.Structure):
     _fields_ = [
         ("penalty", _i32), ("neighbours", _i32), ("lambda_", _f64),
         ("delta", _f64), ("group_spacing", _i64), ("resync_every", _i32),
-        ("tile", _i32),
+        ("tile", _i32), ("plain_curvature", _i32),
     ]
 
 
@@ -186,9 +186,11 @@ class Scan:
                                    _p(self.slice_locations), _p(self.slice_index)))
         self.geom, self.run = g, rp
         self.tile = 0
+        self.plain_curvature = 0
         for line in open(self.prm_path):
-            if line.split(":")[0].strip() == "tile":
-                self.tile = int(line.split(":", 1)[1].split("#")[0])
+            key = line.split(":")[0].strip()
+            if key in ("tile", "plain_curvature"):
+                setattr(self, key, int(line.split(":", 1)[1].split("#")[0]))
 
     def synthesize(self, use_gpu: bool = False, weights: bool | None = None):
         g = self.geom
@@ -213,6 +215,7 @@ class Scan:
         o.group_spacing = kw.get("group_spacing", self.run.group_spacing)
         o.resync_every = kw.get("resync_every", self.run.resync_every)
         o.tile = kw.get("tile", self.tile)
+        o.plain_curvature = kw.get("plain_curvature", self.plain_curvature)
         return o
 
 

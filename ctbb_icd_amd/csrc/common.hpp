@@ -116,6 +116,7 @@ struct ctbb_recon {
     int32_t *gb_idx = nullptr;    // [fov entries] ray index (as in the SSM)
     float *gb_val = nullptr;      // [fov entries] FP32 value (FP32 matrices)
     __half *gb_val16 = nullptr;   // [fov entries] FP16 value (FP16 matrices)
+    float *ssum = nullptr;        // [n_sm*nch*nr] batch surrogate ray sums
     uint8_t *gb_key = nullptr;    // [fov entries] view in block << 6 | rank % 64
     uint16_t *gb_perm = nullptr;  // [nib*ncols_fov] column slot of (b, ib, rank)
     int64_t *gb_ptr = nullptr;    // [chunks+1] start of (b, ib, 64-rank chunk)

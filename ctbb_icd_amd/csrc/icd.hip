@@ -400,6 +400,8 @@ struct GatherArgs {
     float *part1, *part2;     // [ng][nib][nc]
     int64_t nib;
     const int64_t *si_half;   // [nz] central view - n_sm/2
+    const float *ssum;        // batch surrogate: sum of the batch's values
+                              //   per SSM ray (null: sum w a^2)
     int64_t nsc;              // workgroups per view block
     int64_t nsm, readings;
 };
@@ -478,11 +480,12 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                 av[u] = in ? av[u] : 0.0f;
                 r[u] = in ? r[u] : 0;
             }
-            float ev[kGatherUnroll], wv[kGatherUnroll];
+            float ev[kGatherUnroll], wv[kGatherUnroll], sv[kGatherUnroll];
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; u++) {
                 ev[u] = ew[r[u]];
                 wv[u] = (THETA2 && wt) ? wt[r[u]] : 1.0f;
+                sv[u] = (THETA2 && g.ssum) ? g.ssum[iv[u]] : av[u];
             }
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; u++) {
@@ -490,7 +493,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                     break;
                 const uint64_t head = run_heads(kv[u], lane);
                 const float t1 = run_sum(av[u] * ev[u], head, lane);
-                const float t2 = THETA2 ? run_sum((wv[u] * av[u]) * av[u], head, lane) : 0.0f;
+                const float t2 = THETA2 ? run_sum((wv[u] * av[u]) * sv[u], head, lane) : 0.0f;
                 if (ok[u] && run_end(head, lane)) {
                     a1[kv[u]] += t1;
                     if (THETA2)
@@ -576,7 +579,9 @@ __global__ __launch_bounds__(kSweepBlock) void icd_solve_kernel(SweepArgs a,
     if (FP16) {
         const double sc = (double)a.col_scale[col];
         th1 *= sc;
-        th2 *= sc * sc;
+        // sum w a^2 carries the scale twice; the surrogate's ray sums are
+        // formed from scaled values, so sum w a s carries it once
+        th2 *= g.ssum ? sc : sc * sc;
     }
     if (THETA2)
         a.th2[q] = th2;
@@ -1501,6 +1506,7 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
     if (r->ew) (void)hipFree(r->ew);
     if (r->th2) (void)hipFree(r->th2);
     if (r->part1) (void)hipFree(r->part1);
+    if (r->ssum) (void)hipFree(r->ssum);
     if (r->part2) (void)hipFree(r->part2);
     if (r->gb_idx) (void)hipFree(r->gb_idx);
     if (r->gb_val) (void)hipFree(r->gb_val);
@@ -1511,6 +1517,38 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return CTBB_OK;
+}
+
+// Batch surrogate: s[ray] = sum of batch b's matrix values on each SSM ray
+// it touches, from the transposed copy (a (view, channel) segment is sorted
+// by row: runs of one row are contiguous, summed in entry order).  Rays the
+// batch does not touch are not written and never read.
+__global__ void batch_ray_sum_kernel(const uint32_t *bent, const float *bval,
+                                     const int32_t *vc_idx, const int64_t *ent_off,
+                                     int64_t b, int64_t nvc, int64_t nr, float *s)
+{
+    const uint32_t *ent = bent + ent_off[b];
+    const float *val = bval + ent_off[b];
+    const int32_t *vci = vc_idx + b * nvc;
+    const int64_t nseg = nvc - 1;
+    for (int64_t seg = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; seg < nseg;
+         seg += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t lo = vci[seg], hi = vci[seg + 1];
+        int64_t cur = -1;
+        float acc = 0.0f;
+        for (int64_t p = lo; p < hi; p++) {
+            const int64_t row = bt_row(ent[p]);
+            if (row != cur) {
+                if (cur >= 0)
+                    s[seg * nr + cur] = acc;
+                cur = row;
+                acc = 0.0f;
+            }
+            acc += val[p];
+        }
+        if (cur >= 0)
+            s[seg * nr + cur] = acc;
+    }
 }
 
 // View-blocked batch copy (gather).  Per batch b and view block ib the
@@ -2034,6 +2072,13 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
             (rc = dev_alloc_n(&r->th2, r->nvox, "theta2")))
             return bail(rc);
         r->th2_valid = false;
+        // batch surrogate ray sums (one batch at a time, first sweep only);
+        // single-column batches: the surrogate is sum w a^2 itself
+        const bool single = tile >= std::max(g->nx, g->ny);
+        if (!opts->plain_curvature && !single &&
+            (rc = dev_alloc_n(&r->ssum, g->n_views_sm * g->n_channels * g->n_rows,
+                              "surrogate ray sums")))
+            return bail(rc);
         // scatter dispatch stride: median view distance of consecutive members
         r->view_stride.assign(std::max<int64_t>(1, r->spacing), 1);
         for (int64_t j = 0; j < r->spacing && j < g->nz; j++) {
@@ -2215,6 +2260,7 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
     ga.part2 = r->part2;
     ga.nib = r->nib;
     ga.si_half = r->si_half;
+    ga.ssum = r->ssum;
     ga.nsm = r->g.n_views_sm;
     ga.readings = r->g.readings;
     BtArgs bt;
@@ -2260,6 +2306,13 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
         int64_t step = 0, used = 0;
         for (int64_t b = 0; b < r->nbatch; b++) {
             const int64_t c0 = r->h_batch_ptr[b], nc = r->h_batch_ptr[b + 1] - c0;
+            if (first && r->ssum) {
+                const int64_t nvc = r->g.n_views_sm * r->g.n_channels + 1;
+                hipLaunchKernelGGL(batch_ray_sum_kernel,
+                                   dim3((unsigned)std::min<int64_t>(8192, (nvc + 255) / 256)),
+                                   dim3(256), 0, r->stream, r->bt_ent, r->bt_val, r->bt_vc,
+                                   r->bt_off, b, nvc, r->g.n_rows, r->ssum);
+            }
             for (int64_t j = 0; j < r->spacing && j < r->g.nz; j++) {
                 StepArgs sa;
                 sa.c0 = c0;

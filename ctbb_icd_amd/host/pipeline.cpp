@@ -197,6 +197,7 @@ void icd_iteration(const struct recon_params *rp, struct ct_data *data)
     o.group_spacing = (int64_t)rp->group_spacing;
     o.resync_every = (int32_t)rp->resync_every;
     o.tile = (int32_t)rp->tile;
+    o.plain_curvature = (int32_t)rp->plain_curvature;
     ctbb_recon *r = nullptr;
     if (ctbb_recon_create(data->ssm, data->raw, data->weights, si.data(), &o, &r) !=
         CTBB_OK)

@@ -251,6 +251,7 @@ int parse_config_checked(const std::string &config_file, recon_params *rp)
     Q(resync_every, size_t);
     Q(group_spacing, size_t);
     Q(tile, size_t);
+    Q(plain_curvature, size_t);
 #undef Q
     return CTBB_OK;
 }
@@ -284,6 +285,7 @@ static void set_extension_defaults(recon_params *rp)
     rp->resync_every = 0;
     rp->group_spacing = 0;
     rp->tile = 0;
+    rp->plain_curvature = 0;
 }
 
 int configure_recon_params_checked(const std::string &filename, recon_params *out)

@@ -116,6 +116,7 @@ struct recon_params {
     size_t resync_every;           // FP64 resync period in iterations
     size_t group_spacing;          // 0: derive from the SSM footprints
     size_t tile;                   // batch tile of the sweep (0: default)
+    size_t plain_curvature;        // 1: no batch surrogate (ctbb_icd_opts)
     std::string prm_path;          // the parameter file itself
 };
 

@@ -163,6 +163,19 @@ typedef struct ctbb_icd_opts {
                                  start of the step).  0 = 8; tile >=
                                  max(nx, ny) gives single-column batches,
                                  i.e. the reference's raster order. */
+    int32_t plain_curvature;  /* 0 (default): batch surrogate -- the
+                                 curvature of a voxel's update is
+                                 sum_i w_i a_i s_i, s_i = sum over the
+                                 batch's columns of their values on ray i,
+                                 which majorises the data term of the whole
+                                 batch separably, so every step lowers the
+                                 cost however strongly the batch's columns
+                                 share rays (no reference counterpart; equal
+                                 to the reference's sum_i w_i a_i^2 for
+                                 single-column batches).  1: sum_i w_i a_i^2
+                                 in every batch (plain Jacobi across the
+                                 batch; can overshoot when the weighted
+                                 data term dominates the penalty). */
 } ctbb_icd_opts;
 
 typedef struct ctbb_icd_stats {

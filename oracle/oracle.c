@@ -358,13 +358,15 @@ double orc_edge_preserving(const ctbb_geom *g, int64_t q, double alpha,
 /* ------------------------------------------------------------------------ */
 /* ICD: src/icd_iteration.cpp:149-274 (plus optional weights and the        */
 /* batched z-group order).                                                   */
-/* theta sums and penalty step of voxel (c, k) against the current x, yhat */
+/* theta sums and penalty step of voxel (c, k) against the current x, yhat;
+ * with ssum (batch surrogate, no reference counterpart) the curvature is
+ * sum w a ssum[ray] instead of sum w a^2 */
 static double orc_voxel_step(const ctbb_geom *g, const int64_t *col_ptr,
                              const int32_t *idx, const float *val,
                              const float *raw, const float *weights, int64_t c,
                              int64_t k, int64_t off, int penalty,
                              double lambda, double delta, const double *x,
-                             const double *yhat)
+                             const double *yhat, const double *ssum)
 {
     int64_t data_size = g->readings * g->n_channels * g->n_rows;
     int64_t q = c + g->nx * g->ny * k;
@@ -374,7 +376,7 @@ static double orc_voxel_step(const ctbb_geom *g, const int64_t *col_ptr,
         if (r > -1 && r < data_size) {
             double a = val[p];
             double w = weights ? (double)weights[r] : 1.0;
-            alpha += w * a * a;
+            alpha += w * a * (ssum ? ssum[idx[p]] : a);
             beta += w * a * ((double)raw[r] - yhat[r]);
         }
     }
@@ -408,6 +410,17 @@ int64_t orc_icd(const ctbb_geom *g, const int64_t *col_ptr,
                 int penalty, double lambda, double delta, int64_t spacing,
                 int64_t tile, int n_iter, double *x, double *yhat)
 {
+    return orc_icd_ex(g, col_ptr, idx, val, raw, weights, slice_index, penalty,
+                      lambda, delta, spacing, tile, 0, n_iter, x, yhat);
+}
+
+int64_t orc_icd_ex(const ctbb_geom *g, const int64_t *col_ptr,
+                   const int32_t *idx, const float *val, const float *raw,
+                   const float *weights, const int64_t *slice_index,
+                   int penalty, double lambda, double delta, int64_t spacing,
+                   int64_t tile, int surrogate, int n_iter, double *x,
+                   double *yhat)
+{
     int64_t updates = 0;
     double fov_limit = (g->acquisition_fov / 2.0) * (g->acquisition_fov / 2.0);
     int64_t *offs = (int64_t *)malloc(sizeof(int64_t) * g->nz);
@@ -428,7 +441,7 @@ int64_t orc_icd(const ctbb_geom *g, const int64_t *col_ptr,
                         double du = orc_voxel_step(g, col_ptr, idx, val, raw,
                                                    weights, c, k, offs[k],
                                                    penalty, lambda, delta, x,
-                                                   yhat);
+                                                   yhat, NULL);
                         orc_apply(g, col_ptr, idx, val, c, k, offs[k], du, x,
                                   yhat);
                         updates++;
@@ -444,6 +457,12 @@ int64_t orc_icd(const ctbb_geom *g, const int64_t *col_ptr,
     int64_t ntx = (g->nx + tile - 1) / tile, nty = (g->ny + tile - 1) / tile;
     int64_t *bcols = (int64_t *)malloc(sizeof(int64_t) * ntx * nty);
     double *dus = (double *)malloc(sizeof(double) * ntx * nty * g->nz);
+    /* batch surrogate: ssum[i] = sum of the batch's matrix values on SSM ray
+     * i; curvature sum_i w a_ic ssum_i >= sum_i w a_ic^2 majorises the data
+     * term of the whole batch separably (columns of a batch are never
+     * penalty neighbours), so every step decreases the cost */
+    int64_t nray = g->n_views_sm * g->n_channels * g->n_rows;
+    double *ssum = surrogate ? (double *)malloc(sizeof(double) * nray) : NULL;
     for (int n = 0; n < n_iter; n++) {
         for (int64_t ly = 0; ly < tile; ly++)
             for (int64_t lx = 0; lx < tile; lx++) {
@@ -460,6 +479,12 @@ int64_t orc_icd(const ctbb_geom *g, const int64_t *col_ptr,
                     }
                 if (nb == 0)
                     continue;
+                if (ssum) {
+                    memset(ssum, 0, sizeof(double) * nray);
+                    for (int64_t b = 0; b < nb; b++)
+                        for (int64_t p = col_ptr[bcols[b]]; p < col_ptr[bcols[b] + 1]; p++)
+                            ssum[idx[p]] += val[p];
+                }
                 for (int64_t jj = 0; jj < spacing; jj++) {
                     int64_t t = 0;
                     for (int64_t b = 0; b < nb; b++)
@@ -467,7 +492,7 @@ int64_t orc_icd(const ctbb_geom *g, const int64_t *col_ptr,
                             dus[t++] = orc_voxel_step(g, col_ptr, idx, val, raw,
                                                       weights, bcols[b], k,
                                                       offs[k], penalty, lambda,
-                                                      delta, x, yhat);
+                                                      delta, x, yhat, ssum);
                     t = 0;
                     for (int64_t b = 0; b < nb; b++)
                         for (int64_t k = jj; k < g->nz; k += spacing) {
@@ -480,6 +505,7 @@ int64_t orc_icd(const ctbb_geom *g, const int64_t *col_ptr,
     }
     free(bcols);
     free(dus);
+    free(ssum);
     free(offs);
     return updates;
 }

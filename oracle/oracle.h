@@ -52,6 +52,15 @@ int64_t orc_icd(const ctbb_geom *g, const int64_t *col_ptr,
                 const float *weights, const int64_t *slice_index,
                 int penalty, double lambda, double delta, int64_t spacing,
                 int64_t tile, int n_iter, double *x, double *yhat);
+/* orc_icd with the batch surrogate option (surrogate != 0, batched order
+ * only): the curvature of voxel (c, k) is sum_i w a_ic * (sum of the batch's
+ * values on SSM ray i) instead of sum_i w a_ic^2. */
+int64_t orc_icd_ex(const ctbb_geom *g, const int64_t *col_ptr,
+                   const int32_t *idx, const float *val, const float *raw,
+                   const float *weights, const int64_t *slice_index,
+                   int penalty, double lambda, double delta, int64_t spacing,
+                   int64_t tile, int surrogate, int n_iter, double *x,
+                   double *yhat);
 
 /* A9: global cost of x (computes a fresh A x in FP64). */
 double orc_cost(const ctbb_geom *g, const int64_t *col_ptr,

@@ -49,6 +49,8 @@ def lib() -> C.CDLL:
             "orc_edge_preserving": ([G, _i64, _f64, _f64, _f64, _f64, _P], _f64),
             "orc_icd": ([G, _P, _P, _P, _P, _P, _P, C.c_int, _f64, _f64, _i64,
                          _i64, C.c_int, _P, _P], _i64),
+            "orc_icd_ex": ([G, _P, _P, _P, _P, _P, _P, C.c_int, _f64, _f64, _i64,
+                            _i64, C.c_int, C.c_int, _P, _P], _i64),
             "orc_cost": ([G, _P, _P, _P, _P, _P, _P, C.c_int, _f64, _f64, _P], _f64),
             "orc_rotate_slices": ([G, _P, _P, _P, C.c_int], None),
             "orc_column_overlaps": ([G, _P, _P, _i64, _i64], C.c_int),
@@ -106,10 +108,11 @@ def forward_project(geom, ssm, slice_index, x):
 
 
 def icd(geom, ssm, raw, weights, slice_index, penalty, lam, delta, spacing,
-        n_iter, x=None, yhat=None, tile=8):
+        n_iter, x=None, yhat=None, tile=8, surrogate=False):
     """Run n_iter oracle iterations; returns (x, yhat, updates).
     spacing 0: reference order; spacing >= 2: batched z-group order with the
-    given tile (tile >= max(nx, ny): single-column batches)."""
+    given tile (tile >= max(nx, ny): single-column batches), optionally with
+    the batch surrogate curvature."""
     cp, idx, val = ssm
     x = np.zeros(geom.nvox, np.float64) if x is None else np.array(x, np.float64)
     if yhat is None:
@@ -118,9 +121,10 @@ def icd(geom, ssm, raw, weights, slice_index, penalty, lam, delta, spacing,
         yhat = np.array(yhat, np.float64)
     raw = np.ascontiguousarray(raw, np.float32)
     w = None if weights is None else np.ascontiguousarray(weights, np.float32)
-    n = lib().orc_icd(gp(geom), _p(cp), _p(idx), _p(val), _p(raw), _p(w),
-                      _p(np.ascontiguousarray(slice_index, np.int64)), penalty, lam,
-                      delta, spacing, tile, n_iter, _p(x), _p(yhat))
+    n = lib().orc_icd_ex(gp(geom), _p(cp), _p(idx), _p(val), _p(raw), _p(w),
+                         _p(np.ascontiguousarray(slice_index, np.int64)), penalty, lam,
+                         delta, spacing, tile, int(bool(surrogate)), n_iter, _p(x),
+                         _p(yhat))
     return x, yhat, n
 
 

@@ -347,3 +347,47 @@ def test_sweeps_never_increase_cost(make_scan, orc, name):
         c = orc.cost(g, ssm, raw, w, s.slice_index, pen, lam, delta, x)
         assert c <= prev * (1 + 1e-12)
         prev = c
+
+
+def _batched_costs(orc, s, ssm, raw, w, tile, surrogate, sweeps):
+    g = s.geom
+    pen, lam, delta = s.run.penalty, s.run.lambda_, s.run.delta
+    x, yhat = None, None
+    costs = [orc.cost(g, ssm, raw, w, s.slice_index, pen, lam, delta, np.zeros(g.nvox))]
+    for _ in range(sweeps):
+        # one slice per group: the batch's columns are the only coupling
+        x, yhat, _ = orc.icd(g, ssm, raw, w, s.slice_index, pen, lam, delta, g.nz, 1,
+                             x=x, yhat=yhat, tile=tile, surrogate=surrogate)
+        costs.append(orc.cost(g, ssm, raw, w, s.slice_index, pen, lam, delta, x))
+    return np.array(costs), x
+
+
+@pytest.mark.parametrize("name", ["tiny", "tiny_ffs"])
+def test_batch_surrogate_never_increases_cost(make_scan, orc, name):
+    """Batch surrogate (no reference counterpart): the curvature
+    sum w a (sum of the batch's values per ray) majorises the weighted data
+    term of a whole batch, so every batched sweep lowers the cost.  The
+    weighted case with columns 2 apart is one where plain Jacobi curvature
+    (sum w a^2) overshoots and raises the cost -- so the test would catch a
+    surrogate that lost the other columns' values."""
+    s = make_scan(name, synthetic_I0=1e5, synthetic_weights=1)
+    ssm = _ssm(orc, s)
+    raw, w = s.synthesize(use_gpu=False)
+    assert w is not None
+    c_sur, _ = _batched_costs(orc, s, ssm, raw, w, 2, True, 6)
+    assert np.all(np.diff(c_sur) <= 1e-12 * c_sur[0]), c_sur
+    c_plain, _ = _batched_costs(orc, s, ssm, raw, w, 2, False, 6)
+    assert np.any(np.diff(c_plain) > 0), c_plain
+    assert c_sur[-1] < c_plain[-1]
+
+
+def test_batch_surrogate_is_plain_for_single_columns(make_scan, orc):
+    """With single-column batches the batch's value on a ray is the column's
+    own, so the surrogate equals the reference's sum w a^2 exactly."""
+    s = make_scan("tiny_ffs")
+    ssm = _ssm(orc, s)
+    raw, w = s.synthesize(use_gpu=False)
+    t = max(s.geom.nx, s.geom.ny)
+    c1, x1 = _batched_costs(orc, s, ssm, raw, w, t, True, 2)
+    c0, x0 = _batched_costs(orc, s, ssm, raw, w, t, False, 2)
+    assert np.array_equal(x1, x0) and np.array_equal(c1, c0)
