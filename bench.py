@@ -182,7 +182,7 @@ def main():
                         f"{'FP16' if args.fp16 else 'FP32'} SSM, "
                         f"{'edge-preserving' if scan.run.penalty else 'quadratic'} penalty",
             "voxels": g.nvox, "ssm_nnz": nnz, "slice_spacing": spacing,
-            "tile": args.tile or 8,
+            "tile": st.tile, "batch_curvature": "surrogate",
             "parallelism": "1 GPU",
         },
         "roofline": {
@@ -190,7 +190,7 @@ def main():
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": (measured_traffic(args.config, dom)
-                        if not args.fp16 and args.tile in (0, 8) else None),
+                        if not args.fp16 and args.tile == 0 else None),
             "algorithmic_bytes_per_launch": dom_bytes,
             "avg_launch_ms": dom_ms,
         },

@@ -78,6 +78,7 @@ class IcdStats(C.Structure):
         ("sweep_ms", _f64), ("resync_ms", _f64), ("cost", _f64),
         ("sweep_launches", _i64), ("ssm_entries", _i64),
         ("gather_ms_avg", _f64), ("scatter_ms_avg", _f64), ("solve_ms_avg", _f64),
+        ("tile", _i64), ("group_spacing", _i64),
     ]
 
 

@@ -69,9 +69,6 @@ CONFIGS: dict[str, dict] = {
         synthetic_n_ellipsoids=10, synthetic_ellipsoid_radius=8.0,
         synthetic_axis_min=0.5, synthetic_axis_max=4.0, synthetic_mu_min=0.15,
         synthetic_mu_max=0.25, synthetic_I0=1e5, synthetic_weights=1,
-        # the weighted data term dominates the penalty here: batches of
-        # columns 16 apart (64 per batch) keep the Jacobi coupling small
-        tile=16,
     ),
     "config1": dict(
         Readings=6912, n_channels=736, num_views_per_turn_without_ffs=1152,
@@ -100,11 +97,9 @@ CONFIGS["micro"] = dict(
 CONFIGS["config2"] = dict(CONFIGS["config1"], ssm_fp16=1, resync_every=2)
 CONFIGS["config3"] = dict(CONFIGS["config1"], Readings=4608, nx=256, ny=256,
                           z_start=-4.0, z_end=-2.05)
-# weights = counts make the data term dominate the penalty: batches of
-# columns 16 apart (as config0) keep the Jacobi coupling of a step small
 CONFIGS["config4"] = dict(CONFIGS["config1"], Readings=13824, z_start=-16.5,
                           z_end=-1.11, synthetic_I0=1e5, synthetic_lung=4000,
-                          synthetic_weights=1, tile=16)
+                          synthetic_weights=1)
 CONFIGS["config4_slab"] = dict(CONFIGS["config4"], Readings=4608, z_start=-4.0,
                                z_end=-2.05)
 

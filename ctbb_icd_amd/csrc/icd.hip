@@ -1951,7 +1951,14 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     // apart (never penalty neighbours).  Batches run in raster order of
     // (ly, lx); with tile >= max(nx, ny) every batch is a single column and
     // the sweep is the reference's raster order.
-    const int64_t tile = opts->tile > 0 ? opts->tile : 8;
+    // default: tile 4, or the smallest tile whose batches hold at most 2^14
+    // columns (the slot width of the transposed copy)
+    int64_t tile = opts->tile;
+    if (tile <= 0) {
+        tile = 4;
+        while (((g->nx + tile - 1) / tile) * ((g->ny + tile - 1) / tile) > (1 << 14))
+            tile++;
+    }
     if (tile < 2)
         return bail(fail(CTBB_E_ARG, "tile must be >= 2"));
     const double fov_limit = (g->acquisition_fov / 2.0) * (g->acquisition_fov / 2.0);
@@ -2412,6 +2419,8 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
         stats->cost = NAN;
         stats->sweep_launches = launches;
         stats->ssm_entries = r->ssm_fov_entries;
+        stats->tile = r->tile;
+        stats->group_spacing = r->spacing;
     }
     return CTBB_OK;
 }

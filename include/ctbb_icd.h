@@ -160,7 +160,9 @@ typedef struct ctbb_icd_opts {
                                  group's footprints are disjoint); across the
                                  batch's columns it is Jacobi (all gather
                                  from the error sinogram as it stood at the
-                                 start of the step).  0 = 8; tile >=
+                                 start of the step).  0 = 4 (or the
+                                 smallest tile with at most 16384 columns
+                                 per batch); tile >=
                                  max(nx, ny) gives single-column batches,
                                  i.e. the reference's raster order. */
     int32_t plain_curvature;  /* 0 (default): batch surrogate -- the
@@ -193,6 +195,8 @@ typedef struct ctbb_icd_stats {
                                  (HIP events around sampled steps)       */
     double scatter_ms_avg;    /* average scatter-launch duration          */
     double solve_ms_avg;      /* average solve-launch duration            */
+    int64_t tile;             /* batch tile in use                        */
+    int64_t group_spacing;    /* slice spacing s of the groups in use     */
 } ctbb_icd_stats;
 
 /* raw: measured line integrals y [readings*n_channels*n_rows], reference
