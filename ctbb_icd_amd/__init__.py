@@ -126,6 +126,7 @@ def lib() -> C.CDLL:
         "ctbb_recon_iterate": ([P, C.c_int, C.POINTER(IcdStats)], C.c_int),
         "ctbb_recon_resync": ([P], C.c_int),
         "ctbb_recon_cost": ([P, P], C.c_int),
+        "ctbb_recon_cost_running": ([P, P], C.c_int),
         "ctbb_recon_get_error": ([P, P], C.c_int),
         "ctbb_cost": ([P, P, P, P, C.POINTER(IcdOpts), P, P], C.c_int),
         "ctbb_rotate_slices": ([C.POINTER(Geom), P, P, P, C.c_int], C.c_int),
@@ -352,6 +353,12 @@ class Recon:
     def cost(self) -> float:
         c = C.c_double()
         check(lib().ctbb_recon_cost(self.h, C.byref(c)))
+        return c.value
+
+    def cost_running(self) -> float:
+        """Cost from the kept FP32 error sinogram (no projection)."""
+        c = C.c_double()
+        check(lib().ctbb_recon_cost_running(self.h, C.byref(c)))
         return c.value
 
     def free(self) -> None:

@@ -917,6 +917,22 @@ __global__ __launch_bounds__(kRedBlock) void weighted_error_kernel(
         ew[i] = (float)((double)w[i] * (double)e[i]);
 }
 
+// Data part of the cost from the kept FP32 error sinogram: sum w e^2, FP64.
+__global__ __launch_bounds__(kRedBlock) void error_sq_kernel(const float *e, const float *w,
+                                                             int64_t n, double *partials)
+{
+    __shared__ double sh[kRedBlock / 64];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kRedBlock) {
+        const double d = (double)e[i];
+        acc += (w ? (double)w[i] : 1.0) * d * d;
+    }
+    const double s = block_sum_f64(acc, sh);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = s;
+}
+
 // Penalty part of the cost: sum over pairs (q, q+d), d in {1, nx-1, nx, nx+1}.
 __global__ __launch_bounds__(kRedBlock) void penalty_cost_kernel(
     const float *x, int64_t nx, int64_t n, int penalty, double delta,
@@ -2098,6 +2114,9 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
             r->view_stride[j] = std::max<int64_t>(1, d[d.size() / 2]);
         }
     }
+    // ray-major copy for the FP64 projections of set_volume / cost / resync
+    if ((rc = ensure_ray_major(m)) != CTBB_OK)
+        return bail(rc);
     *out = r;
     return CTBB_OK;
 }
@@ -2218,6 +2237,24 @@ extern "C" int ctbb_recon_cost(ctbb_recon *r, double *cost)
         return fail(CTBB_E_ARG, "null argument");
     double data = 0.0, pen = 0.0;
     CTBB_TRY(recon_data_term(r, false, &data));
+    CTBB_TRY(recon_penalty_term(r, &pen));
+    if (r->opts.penalty == CTBB_PENALTY_QUADRATIC)
+        *cost = 0.5 * data + 0.5 * r->opts.lambda * pen;
+    else
+        *cost = data + r->opts.lambda * pen;
+    return CTBB_OK;
+}
+
+extern "C" int ctbb_recon_cost_running(ctbb_recon *r, double *cost)
+{
+    if (!r || !cost)
+        return fail(CTBB_E_ARG, "null argument");
+    double data = 0.0, pen = 0.0;
+    const int g = red_grid(r->data_size);
+    hipLaunchKernelGGL(error_sq_kernel, dim3(g), dim3(kRedBlock), 0, r->stream, r->e, r->w,
+                       r->data_size, r->partials);
+    CTBB_HIP(hipGetLastError());
+    CTBB_TRY(sum_partials(r->partials, g, r->stream, &data));
     CTBB_TRY(recon_penalty_term(r, &pen));
     if (r->opts.penalty == CTBB_PENALTY_QUADRATIC)
         *cost = 0.5 * data + 0.5 * r->opts.lambda * pen;

@@ -170,7 +170,10 @@ def warm_run(scan: Scan, ssm: SSM, raw, w, net: UNet, n_early: int, target: floa
     """n_early plain iterations, the network, the cost check, then ICD to the
     target.  The time includes the early iterations, the rotations, the
     network, the projection of the start image and the two cost evaluations
-    of the check; the iteration count includes the early iterations."""
+    of the check (from the kept error sinogram: the plain image's after its
+    last sweep, the learned start's right after its FP64 projection); the
+    iteration count includes the early iterations.  Target checks (cost_fn,
+    default the FP64 cost) are not timed."""
     run = Run("warm")
     r = Recon(ssm, raw, w, scan.slice_index, scan.opts())
     try:
@@ -178,23 +181,23 @@ def warm_run(scan: Scan, ssm: SSM, raw, w, net: UNet, n_early: int, target: floa
         r.iterate(n_early)
         x_plain = r.volume()
         x_warm = to_rotating(scan, predict_fixed(net, to_fixed(scan, x_plain)))
-        c_plain = r.cost()
+        c_plain = r.cost_running()
         r.set_volume(x_warm)
-        c_warm = r.cost()
+        c_warm = r.cost_running()
         run.accepted = bool(c_warm <= c_plain)
         if not run.accepted:
             r.set_volume(x_plain)
         run.iterations = n_early
-        run.costs = [c_warm if run.accepted else c_plain]
+        spent = time.perf_counter() - t0
         log(f"[warmstart] start cost plain {c_plain:.8e} warm {c_warm:.8e} -> "
-            f"{'accepted' if run.accepted else 'rejected'}")
-        if cost_fn is not None:
-            run.costs = [cost_fn(r)]
+            f"{'accepted' if run.accepted else 'rejected'} ({spent:.2f} s)")
+        run.costs = [(cost_fn or (lambda rr: rr.cost()))(r)]
         if run.costs[0] <= target:
             run.reached = True
-            run.seconds = time.perf_counter() - t0
+            run.seconds = spent
             return run
-        return run_to_target(r, target, max_iter, t0, run, log, cost_fn)
+        return run_to_target(r, target, max_iter, time.perf_counter() - spent, run, log,
+                             cost_fn)
     finally:
         r.free()
 

@@ -224,6 +224,14 @@ int ctbb_recon_resync(ctbb_recon *r);
  * with q+d < nx*ny*nz (8-neighbour) and the matching forward half of the
  * 26-neighbour stencil. */
 int ctbb_recon_cost(ctbb_recon *r, double *cost);
+
+/* Cost of the current volume from the kept FP32 error sinogram, without a
+ * projection: sum w e^2 (FP64 sum of the FP32 samples) plus the penalty,
+ * combined as in ctbb_recon_cost.  Equal to ctbb_recon_cost up to the FP32
+ * rounding of e (right after ctbb_recon_set_volume or a resync, e is the
+ * rounded FP64 projection residual); for cheap comparisons of candidate
+ * volumes and for monitoring.  No reference counterpart. */
+int ctbb_recon_cost_running(ctbb_recon *r, double *cost);
 /* Copy the running FP32 error sinogram to the host. */
 int ctbb_recon_get_error(const ctbb_recon *r, float *e);
 

@@ -225,3 +225,21 @@ def test_synthetic_generator_gpu_matches_cpu(gpu, make_scan):
     # Poisson counts come from the same counter-based streams; libm ulps may
     # flip a rare rejection step
     assert np.mean(wa == wb) > 0.999
+
+
+def test_running_cost_matches_fp64_cost(gpu, ssm_pair, orc):
+    """The cost from the kept FP32 error sinogram equals the FP64 cost up to
+    the FP32 rounding of e: right after set_volume and after sweeps."""
+    s, dev, host = ssm_pair("tiny_ffs")
+    raw, w = s.synthesize(use_gpu=False)
+    r = gpu.Recon(dev, raw, w, s.slice_index, s.opts())
+    x0 = s.phantom()
+    r.set_volume(x0)
+    c64, c32 = r.cost(), r.cost_running()
+    print(f"[running-cost] set_volume: fp64 {c64:.10e} running {c32:.10e}", flush=True)
+    assert abs(c32 - c64) <= 1e-6 * abs(c64)
+    r.iterate(2)
+    c64, c32 = r.cost(), r.cost_running()
+    print(f"[running-cost] 2 sweeps: fp64 {c64:.10e} running {c32:.10e}", flush=True)
+    assert abs(c32 - c64) <= 1e-5 * abs(c64)
+    r.free()
