@@ -604,12 +604,11 @@ __global__ __launch_bounds__(kSweepBlock) void icd_solve_kernel(SweepArgs a,
 // that view (sorted by channel, row, then column slot) give the rays and
 // their column slots.  Wavefront w owns the channels [c0 + 8w, c0 + 8w + 8)
 // of every view of the workgroup: one (view, slice) item's entries for them
-// are contiguous (typically under 64), so one 64-lane chunk covers an item.
-// Lanes form du[slot] * a, runs of equal (channel, row) are summed with
-// shuffles and the last lane of a run adds it into LDS.  The entry loads of
-// kItemUnroll items are issued before any of them is reduced, and items are
-// applied in a fixed order by the one wavefront that owns the channels: no
-// LDS atomics, deterministic sums.  The workgroup then writes each changed
+// are contiguous and are taken in 64-entry chunks.  Lanes form du[slot] * a,
+// runs of equal (channel, row) are summed with shuffles and the last lane of
+// a run adds it into LDS.  The loads of kItemUnroll chunks are issued before
+// any of them is reduced, and chunks are applied in a fixed order by the one
+// wavefront that owns the channels: no LDS atomics, deterministic sums.  The workgroup then writes each changed
 // sample once; no two workgroups touch the same sample, so the scatter is
 // conflict-free.
 // Tile and unroll sizes (overridable at build time for tuning sweeps).
@@ -714,16 +713,36 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
         uint64_t todo = __ballot(hi > lo);
         if (todo && lane == 0)
             any_work = 1;
-        while (todo) {
+        // (item, 64-entry chunk) pairs in item order, kItemUnroll at a time:
+        // the loads of several chunks -- of one long item or of several
+        // short ones -- are in flight before any is reduced; chunks are
+        // applied in stream order (deterministic)
+        int cur = -1;
+        int64_t clo = 0, chi = 0;
+        while (true) {
             int it[kItemUnroll];
             int64_t lo_u[kItemUnroll], hi_u[kItemUnroll];
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
-                it[u] = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
-                todo &= todo ? todo - 1 : 0;
-                lo_u[u] = __shfl(lo, it[u] < 0 ? 0 : it[u], 64);
-                hi_u[u] = it[u] < 0 ? lo_u[u] : __shfl(hi, it[u], 64);
+                if (clo >= chi && todo) {
+                    cur = __ffsll((unsigned long long)todo) - 1;
+                    todo &= todo - 1;
+                    clo = __shfl(lo, cur, 64);
+                    chi = __shfl(hi, cur, 64);
+                }
+                if (clo < chi) {
+                    it[u] = cur;
+                    lo_u[u] = clo;
+                    hi_u[u] = min(clo + 64, chi);
+                    clo += 64;
+                } else {
+                    it[u] = -1;
+                    lo_u[u] = 0;
+                    hi_u[u] = 0;
+                }
             }
+            if (it[0] < 0)
+                break;
             uint32_t ev[kItemUnroll];
             float av[kItemUnroll];
             bool ok[kItemUnroll];
@@ -744,17 +763,8 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
             for (int u = 0; u < kItemUnroll; u++) {
                 if (it[u] < 0)
                     continue;
-                float *tacc = acc + (it[u] / kItemLanes) * tile;
-                scatter_chunk(ev[u], dv[u], ok[u], c0, nr, lane, tacc);
-                // items with more than 64 entries on these channels
-                const float *du = a.du + (vb + (it[u] % kItemLanes)) * sa.nc;
-                for (int64_t p0 = lo_u[u] + 64; p0 < hi_u[u]; p0 += 64) {
-                    const int64_t p = p0 + lane;
-                    const bool okp = p < hi_u[u];
-                    const uint32_t e0 = okp ? ent[p] : 0u;
-                    const float d = okp ? du[bt_slot(e0)] * val[p] : 0.0f;
-                    scatter_chunk(e0, d, okp, c0, nr, lane, tacc);
-                }
+                scatter_chunk(ev[u], dv[u], ok[u], c0, nr, lane,
+                              acc + (it[u] / kItemLanes) * tile);
             }
         }
     }
