@@ -107,6 +107,7 @@ def lib() -> C.CDLL:
         "ctbb_copy_probe": ([_i64, C.POINTER(_f64)], C.c_int),
         "ctbb_configure": ([C.c_char_p, C.POINTER(Geom), C.POINTER(RunParams), P, P, P, P], C.c_int),
         "ctbb_synthesize": ([C.c_char_p, P, P, C.c_int], C.c_int),
+        "ctbb_read_raw": ([C.c_char_p, P], C.c_int),
         "ctbb_phantom_volume": ([C.c_char_p, P], C.c_int),
         "ctbb_ssm_build": ([C.POINTER(Geom), C.POINTER(P)], C.c_int),
         "ctbb_ssm_to_fp16": ([P], C.c_int),
@@ -202,6 +203,12 @@ class Scan:
         check(lib().ctbb_synthesize(self.prm_path.encode(), _p(raw), _p(w),
                                     1 if use_gpu else 0))
         return raw, w
+
+    def read_raw(self) -> np.ndarray:
+        """FileType-0 binary frames of the prm's sinogram_path."""
+        raw = np.zeros(self.geom.data_size, np.float32)
+        check(lib().ctbb_read_raw(self.prm_path.encode(), _p(raw)))
+        return raw
 
     def phantom(self) -> np.ndarray:
         v = np.zeros(self.geom.nvox, np.float32)

@@ -24,7 +24,11 @@
 // e -= du a, sample-driven and conflict-free.  Across the batch's columns the
 // step is Jacobi, along a column Gauss-Seidel in group order; with tile >=
 // max(nx, ny) batches are single columns and the order is the reference's.
-// The oracle replays exactly this order.
+// By default the curvature of a voxel's update is the batch surrogate
+// sum_i w_i a_i s_i (s_i: the batch's summed matrix values on ray i), which
+// majorises the batch's data term, so every step lowers the cost; for single
+// columns it is the reference's sum_i w_i a_i^2.  The oracle replays exactly
+// this order and curvature.
 //
 // FP64 is used for the per-voxel sums of partials and the penalty solve, the
 // forward projection of the periodic resync e = y - A x, and the cost.
@@ -364,7 +368,8 @@ struct StepArgs {
 // block (channel, then row), in chunks of 64 ranks; inside a chunk the
 // entries are ordered (view in block, rank).  A workgroup takes one view
 // block and 4 chunks (one per wavefront) and loops over the members of the
-// slice group: the chunk's entries are read once per member from L1/L2, and
+// slice group, kGatherMembers of them per pass over the chunk's entries
+// (each entry is loaded once per pass and gathered at each member's sample);
 // neighbouring lanes read neighbouring samples of the error sinogram, so a
 // 64-lane gather touches few cache lines.  Lanes form a * (w e) at sample
 // idx + slice offset (and, on the first sweep, w a^2); runs of equal entry
@@ -382,10 +387,18 @@ struct StepArgs {
 #define CTBB_VIEW_BLOCK 4
 #endif
 #ifndef CTBB_GATHER_UNROLL
-#define CTBB_GATHER_UNROLL 4
+#define CTBB_GATHER_UNROLL 2
+#endif
+#ifndef CTBB_GATHER_MEMBERS
+#define CTBB_GATHER_MEMBERS 4
 #endif
 constexpr int kViewBlock = CTBB_VIEW_BLOCK;
 constexpr int kGatherUnroll = CTBB_GATHER_UNROLL;
+// Members of the slice group a wavefront takes per pass over its entries in
+// the later sweeps (>= 2: the first sweep uses the second accumulator for
+// the theta2 sums)
+constexpr int kGatherMembers = CTBB_GATHER_MEMBERS;
+static_assert(kGatherMembers >= 2, "two LDS accumulators are needed");
 
 struct GatherArgs {
     const int32_t *idx;       // view-blocked copy (see above)
@@ -429,7 +442,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                                                                  GatherArgs g)
 {
     constexpr int kAcc = 64 * kViewBlock;
-    __shared__ float acc1[kWaves * kAcc], acc2[kWaves * kAcc];
+    __shared__ float accm[kGatherMembers][kWaves * kAcc];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t nwg = g.nib * g.nsc;
     const int64_t per_xcd = (nwg + kXcd - 1) / kXcd;
@@ -445,10 +458,88 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
     const int64_t rk = w * 64 + lane;
     const bool own = rk < sa.nc;
     const int64_t slot = own ? (int64_t)g.perm[g.base + ib * sa.nc + rk] : 0;
-    float *a1 = acc1 + wave * kAcc, *a2 = acc2 + wave * kAcc;
+    float *a1 = accm[0] + wave * kAcc, *a2 = accm[1] + wave * kAcc;
     const float *__restrict__ ew = a.ew;
     const float *__restrict__ wt = a.w;
     const int64_t ds = a.data_size;
+    if (!THETA2 && kGatherMembers > 1) {
+        // later sweeps: kGatherMembers members of the slice group per pass
+        // over the chunk's entries, each with its own LDS accumulators, so
+        // every entry is loaded once for all of them and that many times as
+        // many gathers are in flight
+        constexpr int M = kGatherMembers;
+        int64_t v = 0;
+        while (true) {
+            int64_t vm[M];
+            bool has[M];
+            int64_t off[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                while (v < sa.ng && !block_valid(ib, g.si_half[sa.j + a.spacing * v], g.nsm,
+                                                 g.readings))
+                    v++;
+                has[m] = v < sa.ng;
+                vm[m] = v;
+                off[m] = has[m] ? a.slice_off[sa.j + a.spacing * v] : 0;
+                if (has[m])
+                    v++;
+            }
+            if (!has[0])
+                break;
+#pragma unroll
+            for (int m = 0; m < M; m++)
+#pragma unroll
+                for (int q = 0; q < kViewBlock; q++)
+                    accm[m][wave * kAcc + q * 64 + lane] = 0.0f;
+            for (int64_t p0 = lo; p0 < hi; p0 += 64 * kGatherUnroll) {
+                int32_t iv[kGatherUnroll], kv[kGatherUnroll];
+                float av[kGatherUnroll];
+#pragma unroll
+                for (int u = 0; u < kGatherUnroll; u++) {
+                    const int64_t p = p0 + u * 64 + lane;
+                    const bool ok = p < hi;
+                    iv[u] = ok ? g.idx[p] : 0;
+                    av[u] = ok ? gb_value<FP16>(g, p) : 0.0f;
+                    kv[u] = ok ? (int32_t)g.key[p] : -1;
+                }
+                float ev[M][kGatherUnroll], mv[M][kGatherUnroll];
+#pragma unroll
+                for (int m = 0; m < M; m++)
+#pragma unroll
+                    for (int u = 0; u < kGatherUnroll; u++) {
+                        const int64_t r = (int64_t)iv[u] + off[m];
+                        const bool in = has[m] && kv[u] >= 0 && r >= 0 && r < ds;
+                        mv[m][u] = in ? av[u] : 0.0f;
+                        ev[m][u] = ew[in ? r : 0];
+                    }
+#pragma unroll
+                for (int u = 0; u < kGatherUnroll; u++) {
+                    if (p0 + u * 64 >= hi)
+                        break;
+                    const uint64_t head = run_heads(kv[u], lane);
+                    float t[M];
+#pragma unroll
+                    for (int m = 0; m < M; m++)
+                        t[m] = run_sum(mv[m][u] * ev[m][u], head, lane);
+                    if (kv[u] >= 0 && run_end(head, lane)) {
+#pragma unroll
+                        for (int m = 0; m < M; m++)
+                            accm[m][wave * kAcc + kv[u]] += t[m];
+                    }
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                float sm = 0.0f;
+#pragma unroll
+                for (int q = 0; q < kViewBlock; q++)
+                    sm += accm[m][wave * kAcc + q * 64 + lane];
+                if (own && has[m])
+                    g.part1[(vm[m] * g.nib + ib) * sa.nc + slot] = sm;
+            }
+        }
+        return;
+    }
     for (int64_t v = 0; v < sa.ng; v++) {
         const int64_t k = sa.j + a.spacing * v;
         if (!block_valid(ib, g.si_half[k], g.nsm, g.readings))
@@ -872,29 +963,92 @@ __global__ void bt_fill_kernel(const int64_t *col_ptr, const int32_t *idx,
     }
 }
 
-// Sort each (view, channel) segment by (row, slot): a deterministic order.
-__global__ void bt_sort_kernel(const int32_t *vc_idx, const int64_t *ent_off,
-                               int64_t nbatch, int64_t nvc, uint32_t *ent,
-                               float *bval)
+// Rank placement of a segment of up to 64 * NC entries held in registers
+// (NC chunks of 64 lanes): each entry goes to the number of smaller keys,
+// counted against every key of the segment by lane broadcasts.
+template <int NC>
+__device__ void wave_rank_place(int64_t lo, int64_t n, uint32_t *key, float *val, int lane)
 {
+    uint32_t k[NC];
+    float v[NC];
+    int rank[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int64_t i = c * 64 + lane;
+        k[c] = i < n ? key[lo + i] : 0xffffffffu;
+        v[c] = i < n ? val[lo + i] : 0.0f;
+        rank[c] = 0;
+    }
+    const int nch = (int)((n + 63) / 64);
+    for (int c2 = 0; c2 < nch; c2++) {
+        uint32_t kc = k[0];
+#pragma unroll
+        for (int c = 1; c < NC; c++)
+            kc = c == c2 ? k[c] : kc;
+        for (int j = 0; j < 64; j++) {
+            const uint32_t kj = (uint32_t)__builtin_amdgcn_readlane((int)kc, j);
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+                rank[c] += kj < k[c] ? 1 : 0;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int64_t i = c * 64 + lane;
+        if (i < n) {
+            key[lo + rank[c]] = k[c];
+            val[lo + rank[c]] = v[c];
+        }
+    }
+}
+
+// Sort one segment [lo, hi) of (key, value) pairs by key -- keys unique in
+// the segment, below 2^32 - 1 -- with one wavefront, by rank placement for
+// up to 1024 entries and by an insertion sort of lane 0 beyond that.  The
+// result does not depend on the input order.
+__device__ void wave_rank_sort(int64_t lo, int64_t hi, uint32_t *key, float *val, int lane)
+{
+    const int64_t n = hi - lo;
+    if (n <= 1)
+        return;
+    if (n <= 128) {
+        wave_rank_place<2>(lo, n, key, val, lane);
+        return;
+    }
+    if (n <= 1024) {
+        wave_rank_place<16>(lo, n, key, val, lane);
+        return;
+    }
+    if (lane == 0)
+        for (int64_t i = lo + 1; i < hi; i++) {
+            const uint32_t ke = key[i];
+            const float kv = val[i];
+            int64_t jj = i - 1;
+            while (jj >= lo && key[jj] > ke) {
+                key[jj + 1] = key[jj];
+                val[jj + 1] = val[jj];
+                jj--;
+            }
+            key[jj + 1] = ke;
+            val[jj + 1] = kv;
+        }
+}
+
+// Sort each (view, channel) segment by (row, slot): a deterministic order.
+// One wavefront per segment.
+__global__ __launch_bounds__(256) void bt_sort_kernel(const int32_t *vc_idx,
+                                                      const int64_t *ent_off,
+                                                      int64_t nbatch, int64_t nvc,
+                                                      uint32_t *ent, float *bval)
+{
+    const int lane = threadIdx.x & 63;
     const int64_t nseg = nbatch * (nvc - 1);
-    for (int64_t sg = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; sg < nseg;
-         sg += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t sg = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; sg < nseg;
+         sg += (int64_t)gridDim.x * blockDim.x / 64) {
         const int64_t b = sg / (nvc - 1), key = sg - b * (nvc - 1);
         const int64_t lo = ent_off[b] + vc_idx[b * nvc + key];
         const int64_t hi = ent_off[b] + vc_idx[b * nvc + key + 1];
-        for (int64_t i = lo + 1; i < hi; i++) {
-            const uint32_t ke = ent[i];
-            const float kv = bval[i];
-            int64_t jj = i - 1;
-            while (jj >= lo && ent[jj] > ke) {
-                ent[jj + 1] = ent[jj];
-                bval[jj + 1] = bval[jj];
-                jj--;
-            }
-            ent[jj + 1] = ke;
-            bval[jj + 1] = kv;
-        }
+        wave_rank_sort(lo, hi, ent, bval, lane);
     }
 }
 
@@ -1089,26 +1243,15 @@ __global__ void rm_fill_kernel(const int64_t *col_ptr, const int32_t *idx,
         }
 }
 
-// Insertion sort of every ray by column (the fill order is arbitrary).
-__global__ void rm_sort_kernel(const int64_t *rptr, int64_t nrays, uint32_t *rent,
-                               float *rval)
+// Sort every ray's entries by column (the fill order is arbitrary): one
+// wavefront per ray.
+__global__ __launch_bounds__(256) void rm_sort_kernel(const int64_t *rptr, int64_t nrays,
+                                                      uint32_t *rent, float *rval)
 {
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nrays;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t lo = rptr[q], hi = rptr[q + 1];
-        for (int64_t i = lo + 1; i < hi; i++) {
-            const uint32_t kc = rent[i];
-            const float kv = rval[i];
-            int64_t j = i - 1;
-            while (j >= lo && rent[j] > kc) {
-                rent[j + 1] = rent[j];
-                rval[j + 1] = rval[j];
-                j--;
-            }
-            rent[j + 1] = kc;
-            rval[j + 1] = kv;
-        }
-    }
+    const int lane = threadIdx.x & 63;
+    for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; q < nrays;
+         q += (int64_t)gridDim.x * blockDim.x / 64)
+        wave_rank_sort(rptr[q], rptr[q + 1], rent, rval, lane);
 }
 
 __global__ void rm_group_kernel(const int64_t *rptr, int64_t ngroups, int64_t nr,

@@ -1,8 +1,9 @@
 // pipeline.cpp -- the reference's pipeline functions, same names and
 // argument meaning, running the hot path on the GPU through the C ABI.
 //
-//   load_raw                       src/setup.cpp:245-420 (synthetic data in place
-//                                  of the reader library; FileType-0 tables)
+//   load_raw                       src/setup.cpp:245-420 (FileType-0 binary
+//                                  frames, or seeded synthetic data; FileType-0
+//                                  angle/table conventions)
 //   initialize_recon_volume        src/initialize_recon_volume.cpp:51-76,180-183
 //   generate_system_matrix[_ffs]   src/generate_system_matrix.cpp:54-209,
 //                                  src/generate_system_matrix_ffs.cpp:60-239
@@ -95,19 +96,28 @@ void load_raw(struct recon_params *rp, struct ct_data *data)
 {
     if (!rp->synthetic) {
         std::cout << "Checking for raw file: " << rp->sinogram_path << std::endl;
-        std::cout << "Raw projection readers (FileType 0/1/4 through the "
-                     "ctbb_read library) are not part of this build; set "
-                     "`synthetic: 1` to generate seeded raw data. Exiting"
-                  << std::endl;
-        exit(1);
+        if (rp->FileType != 0) {
+            std::cout << "Raw projection readers for FileType 1/4 (the ctbb_read "
+                         "library) are not part of this build. Exiting"
+                      << std::endl;
+            exit(1);
+        }
     }
     if (build_slice_table_checked(rp, data) != CTBB_OK)
         exit(1);
     const size_t n = rp->Readings * rp->n_channels * rp->Nrows_projection;
     data->raw = new float[n];
-    data->weights = rp->synthetic_weights ? new float[n] : nullptr;
     const size_t nv = rp->num_voxels_x * rp->num_voxels_y * rp->num_voxels_z;
     data->recon_volume = new float[nv]();
+    if (!rp->synthetic) {
+        // binary frames, unweighted as in the reference
+        data->weights = nullptr;
+        if (ctbb_read_raw(rp->prm_path.c_str(), data->raw) != CTBB_OK)
+            die(ctbb_last_error());
+        std::cout << "Read " << rp->Readings << " binary frames" << std::endl;
+        return;
+    }
+    data->weights = rp->synthetic_weights ? new float[n] : nullptr;
     if (ctbb_synthesize(rp->prm_path.c_str(), data->raw, data->weights, 1) != CTBB_OK)
         die("Synthetic raw data generation failed");
     std::cout << "Generated synthetic raw data (" << rp->Readings << " views)"

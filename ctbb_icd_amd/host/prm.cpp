@@ -633,3 +633,37 @@ extern "C" int ctbb_configure(const char *prm_path, ctbb_geom *g,
     delete[] d.slice_indices;
     return CTBB_OK;
 }
+
+// FileType 0 (binary) raw data, src/setup.cpp:271-302 and 335-344: frame i
+// of n_channels x Nrows float32 samples, rows fastest inside a channel, read
+// by ctbb_read's ReadBinaryFrame at RawOffset; the reference transposes the
+// frame to channel-fastest and back again, so the samples keep the file's
+// order: raw[row + Nrows*(channel + n_channels*view)].  ctbb_read is not
+// available here; the frame placement taken (RawOffset bytes of header,
+// frames back to back, native float32) is the plain reading of those calls.
+extern "C" int ctbb_read_raw(const char *prm_path, float *raw)
+{
+    if (!prm_path || !raw)
+        return ctbb::fail(CTBB_E_ARG, "null argument");
+    recon_params rp;
+    int rc = configure_recon_params_checked(prm_path, &rp);
+    if (rc != CTBB_OK)
+        return ctbb::fail(rc, "parameter file rejected (see stdout)");
+    if (rp.FileType != 0)
+        return ctbb::fail(CTBB_E_UNSUPPORTED,
+                          "FileType " + std::to_string(rp.FileType) +
+                              ": the PTR/IMA readers of ctbb_read are not part of this build");
+    FILE *f = fopen(rp.sinogram_path.c_str(), "rb");
+    if (!f)
+        return ctbb::fail(CTBB_E_IO, "raw file not found: " + rp.sinogram_path);
+    const size_t frame = rp.n_channels * rp.Nrows_projection;
+    rc = CTBB_OK;
+    if (fseek(f, (long)rp.RawOffset, SEEK_SET) != 0 ||
+        fread(raw, sizeof(float), frame * rp.Readings, f) != frame * rp.Readings)
+        rc = ctbb::fail(CTBB_E_IO, "raw file " + rp.sinogram_path + " holds fewer than " +
+                                       std::to_string(rp.Readings) + " frames of " +
+                                       std::to_string(frame) + " float32 samples after " +
+                                       std::to_string(rp.RawOffset) + " header bytes");
+    fclose(f);
+    return rc;
+}

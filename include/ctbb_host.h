@@ -56,6 +56,15 @@ int ctbb_configure(const char *prm_path, ctbb_geom *g, ctbb_run_params *p,
 int ctbb_synthesize(const char *prm_path, float *raw, float *weights,
                     int use_gpu);
 
+/* Raw data of a FileType-0 (binary) scan: replaces the binary branch of
+ * load_raw (src/setup.cpp:271-302, 335-344).  Reads prm `Readings` frames
+ * of n_channels x Nrows float32 samples (rows fastest) from `sinogram_path`
+ * after `RawOffset` header bytes, frames back to back; raw
+ * [readings*n_channels*n_rows] in the reference layout (the reference's two
+ * frame transposes cancel).  Other FileTypes (PTR, IMA: the ctbb_read
+ * library) return CTBB_E_UNSUPPORTED; a missing or short file CTBB_E_IO. */
+int ctbb_read_raw(const char *prm_path, float *raw);
+
 /* Ground-truth phantom sampled at the voxel centres of the rotated frame
  * (slice k rotated by its central view's angle), float32 [nx*ny*nz]. */
 int ctbb_phantom_volume(const char *prm_path, float *volume);

@@ -84,3 +84,36 @@ def test_driver_end_to_end(gpu, ct, tmp_path):
     assert np.fromfile(out / "reconstructions" / "iteration0.rcn", np.float64).size == g.nvox
     fixed = ct.rotate_slices(g, s.tube_angles, s.slice_index, x, 1)
     assert np.array_equal(runs[1], fixed)
+
+
+def test_driver_reads_binary_frames(gpu, ct, tmp_path):
+    """The driver on a FileType-0 binary file (synthetic: 0) reconstructs the
+    same volume as the library does from the same frames (unweighted, as the
+    reference is for binary files)."""
+    from ctbb_icd_amd import configs
+    out = tmp_path / "outb"
+    (out / "reconstructions").mkdir(parents=True)
+    s0 = ct.Scan(configs.write_prm("tiny_ffs", tmp_path / "gen.yaml", synthetic_weights=0))
+    raw, _ = s0.synthesize(use_gpu=True)
+    frames = tmp_path / "frames.bin"
+    with open(frames, "wb") as f:
+        f.write(bytes(256))
+        f.write(raw.astype(np.float32).tobytes())
+    prm = configs.write_prm("tiny_ffs", tmp_path / "bin.yaml", output_dir=str(out),
+                            num_iterations=2, synthetic=0, sinogram_path=str(frames),
+                            RawOffset=256)
+    driver = ROOT / "ctbb_icd_amd" / "ctbb_icd"
+    p = subprocess.run([str(driver), str(prm)], capture_output=True, text=True, timeout=120)
+    print(p.stdout[-400:], flush=True)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "binary frames" in p.stdout
+    vol = np.fromfile(out / "reconstruction.img", np.float32)
+    s = ct.Scan(prm)
+    g = s.geom
+    m = ct.SSM.build(g)
+    r = ct.Recon(m, s.read_raw(), None, s.slice_index, s.opts())
+    r.set_volume(np.zeros(g.nvox, np.float32))
+    r.iterate(1)
+    r.iterate(1)
+    fixed = ct.rotate_slices(g, s.tube_angles, s.slice_index, r.volume(), 1)
+    assert np.array_equal(vol, fixed)
