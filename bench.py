@@ -119,6 +119,7 @@ def main():
     over = {"ssm_fp16": 1} if args.fp16 else {}
     scan = ct.Scan(configs.write_prm(args.config, Path(tmp.name) / "bench.yaml", **over))
     g = scan.geom
+    fp16 = bool(args.fp16 or scan.run.ssm_fp16)      # --fp16 or the prm's ssm_fp16
     t0 = time.perf_counter()
     raw, w = scan.synthesize(use_gpu=True)
     t1 = time.perf_counter()
@@ -134,7 +135,7 @@ def main():
         cpu = cpu_baseline(scan, ssm, raw, w, args.cpu_seconds, threads)
         print(f"[bench] cpu baseline: {cpu['value']:.3e} updates/s ({cpu['sample']})",
               file=sys.stderr, flush=True)
-    if args.fp16:
+    if fp16:
         ssm.to_fp16()
     r = ct.Recon(ssm, raw, w, scan.slice_index, scan.opts(tile=args.tile))
     if args.warmup:
@@ -143,7 +144,7 @@ def main():
     st = r.iterate(args.steps)
     wall = time.perf_counter() - t
     weighted = w is not None
-    entry_bytes = 6 if args.fp16 else 8
+    entry_bytes = 6 if fp16 else 8
     steps = st.sweep_launches // 3                 # gather, solve, scatter per step
     visits = st.entry_visits / steps               # in-range entries per step
     # algorithmic bytes per launch (per in-range entry: 4 B error read in the
@@ -173,13 +174,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16-ssm/f32" if args.fp16 else "f32",
+        "dtype": "fp16-ssm/f32" if fp16 else "f32",
         "data": "synthetic (seeded ellipsoid phantom, analytic line integrals, Poisson noise)",
         "config": {
             "workload": f"{args.config}: plain ICD {g.nx}x{g.ny}x{g.nz}, "
                         f"{g.n_rows}x{g.n_channels} detector, {scan.run.num_views_per_turn} "
                         f"views/turn (in-plane FFS), {g.readings} views, "
-                        f"{'FP16' if args.fp16 else 'FP32'} SSM, "
+                        f"{'FP16' if fp16 else 'FP32'} SSM, "
+                        f"{'FP64 resync every %d, ' % scan.run.resync_every if scan.run.resync_every else ''}"
                         f"{'edge-preserving' if scan.run.penalty else 'quadratic'} penalty",
             "voxels": g.nvox, "ssm_nnz": nnz, "slice_spacing": spacing,
             "tile": st.tile, "batch_curvature": "surrogate",
@@ -190,7 +192,7 @@ def main():
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": (measured_traffic(args.config, dom)
-                        if not args.fp16 and args.tile == 0 else None),
+                        if not fp16 and args.tile == 0 else None),
             "algorithmic_bytes_per_launch": dom_bytes,
             "avg_launch_ms": dom_ms,
         },

@@ -472,7 +472,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
         while (true) {
             int64_t vm[M];
             bool has[M];
-            int64_t off[M];
+            int32_t off[M];
 #pragma unroll
             for (int m = 0; m < M; m++) {
                 while (v < sa.ng && !block_valid(ib, g.si_half[sa.j + a.spacing * v], g.nsm,
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                     v++;
                 has[m] = v < sa.ng;
                 vm[m] = v;
-                off[m] = has[m] ? a.slice_off[sa.j + a.spacing * v] : 0;
+                off[m] = has[m] ? (int32_t)a.slice_off[sa.j + a.spacing * v] : 0;
                 if (has[m])
                     v++;
             }
@@ -507,8 +507,9 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                 for (int m = 0; m < M; m++)
 #pragma unroll
                     for (int u = 0; u < kGatherUnroll; u++) {
-                        const int64_t r = (int64_t)iv[u] + off[m];
-                        const bool in = has[m] && kv[u] >= 0 && r >= 0 && r < ds;
+                        // |ray index| + |slice offset| < 2^31 (checked at set-up)
+                        const int32_t r = iv[u] + off[m];
+                        const bool in = has[m] && kv[u] >= 0 && (uint32_t)r < (uint32_t)ds;
                         mv[m][u] = in ? av[u] : 0.0f;
                         ev[m][u] = ew[in ? r : 0];
                     }
@@ -734,10 +735,10 @@ struct BtArgs {
 // One 64-entry chunk of an item: reduce runs of equal samples and add them
 // into the view's tile (acc: rows of nr samples per channel from c0).
 __device__ __forceinline__ void scatter_chunk(uint32_t e0, float d, bool ok,
-                                              int64_t c0, int nr, int lane,
+                                              int c0, int nr, int lane,
                                               float *acc)
 {
-    const int key = ok ? (int)(bt_ch(e0) - c0) * nr + (int)bt_row(e0) : -1;
+    const int key = ok ? ((int)bt_ch(e0) - c0) * nr + (int)bt_row(e0) : -1;
     const uint64_t head = run_heads(key, lane);
     d = run_sum(d, head, lane);
     if (ok && run_end(head, lane))
@@ -793,7 +794,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
     for (int64_t vb = 0; vb < sa.ng; vb += kItemLanes) {
         const int u_l = lane / kItemLanes;
         const int64_t v_l = vb + (lane % kItemLanes);
-        int64_t lo = 0, hi = 0;
+        int32_t lo = 0, hi = 0;
         if (u_l < nview && v_l < sa.ng && wc1 > wc0) {
             const int64_t i = view0 + u_l - bt.si_half[sa.j + a.spacing * v_l];
             if (i >= 0 && i < bt.nsm) {
@@ -809,10 +810,10 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
         // short ones -- are in flight before any is reduced; chunks are
         // applied in stream order (deterministic)
         int cur = -1;
-        int64_t clo = 0, chi = 0;
+        int32_t clo = 0, chi = 0;
         while (true) {
             int it[kItemUnroll];
-            int64_t lo_u[kItemUnroll], hi_u[kItemUnroll];
+            int32_t lo_u[kItemUnroll], hi_u[kItemUnroll];
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
                 if (clo >= chi && todo) {
@@ -839,7 +840,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
             bool ok[kItemUnroll];
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
-                const int64_t p = lo_u[u] + lane;
+                const int32_t p = lo_u[u] + lane;
                 ok[u] = p < hi_u[u];
                 ev[u] = ok[u] ? ent[p] : 0u;
                 av[u] = ok[u] ? val[p] : 0.0f;
@@ -847,14 +848,15 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
             float dv[kItemUnroll];
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
-                const int64_t vm = vb + (it[u] < 0 ? 0 : (it[u] % kItemLanes));
-                dv[u] = ok[u] ? a.du[vm * sa.nc + bt_slot(ev[u])] * av[u] : 0.0f;
+                const int32_t vm = (int32_t)vb + (it[u] < 0 ? 0 : (it[u] % kItemLanes));
+                dv[u] = ok[u] ? a.du[vm * (int32_t)sa.nc + (int32_t)bt_slot(ev[u])] * av[u]
+                              : 0.0f;
             }
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
                 if (it[u] < 0)
                     continue;
-                scatter_chunk(ev[u], dv[u], ok[u], c0, nr, lane,
+                scatter_chunk(ev[u], dv[u], ok[u], (int)c0, nr, lane,
                               acc + (it[u] / kItemLanes) * tile);
             }
         }
@@ -2154,6 +2156,8 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     r->ncols_fov = (int64_t)cols.size();
     r->nbatch = (int64_t)bptr.size() - 1;
     r->tile = tile;
+    if (r->data_size + g->n_views_sm * g->n_channels * g->n_rows >= ((int64_t)1 << 31))
+        return bail(fail(CTBB_E_UNSUPPORTED, "sinogram of 2^31 samples or more"));
     int64_t max_batch = 0;
     for (int64_t b = 0; b < r->nbatch; b++)
         max_batch = std::max(max_batch, bptr[b + 1] - bptr[b]);
