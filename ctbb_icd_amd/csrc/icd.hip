@@ -395,10 +395,11 @@ struct StepArgs {
 constexpr int kViewBlock = CTBB_VIEW_BLOCK;
 constexpr int kGatherUnroll = CTBB_GATHER_UNROLL;
 // Members of the slice group a wavefront takes per pass over its entries in
-// the later sweeps (>= 2: the first sweep uses the second accumulator for
-// the theta2 sums)
+// the later sweeps; the first sweep, which also sums theta2, takes half as
+// many (even, >= 2)
 constexpr int kGatherMembers = CTBB_GATHER_MEMBERS;
-static_assert(kGatherMembers >= 2, "two LDS accumulators are needed");
+static_assert(kGatherMembers >= 2 && kGatherMembers % 2 == 0,
+              "an even number of LDS accumulators is needed");
 
 struct GatherArgs {
     const int32_t *idx;       // view-blocked copy (see above)
@@ -458,153 +459,102 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
     const int64_t rk = w * 64 + lane;
     const bool own = rk < sa.nc;
     const int64_t slot = own ? (int64_t)g.perm[g.base + ib * sa.nc + rk] : 0;
-    float *a1 = accm[0] + wave * kAcc, *a2 = accm[1] + wave * kAcc;
     const float *__restrict__ ew = a.ew;
     const float *__restrict__ wt = a.w;
-    const int64_t ds = a.data_size;
-    if (!THETA2 && kGatherMembers > 1) {
-        // later sweeps: kGatherMembers members of the slice group per pass
-        // over the chunk's entries, each with its own LDS accumulators, so
-        // every entry is loaded once for all of them and that many times as
-        // many gathers are in flight
-        constexpr int M = kGatherMembers;
-        int64_t v = 0;
-        while (true) {
-            int64_t vm[M];
-            bool has[M];
-            int32_t off[M];
+    const uint32_t ds = (uint32_t)a.data_size;
+    // M members of the slice group per pass over the chunk's entries, each
+    // with its own LDS accumulators (the first sweep also sums theta2 and
+    // takes half as many members), so every entry is loaded once for all of
+    // them and M times as many gathers are in flight
+    constexpr int M = THETA2 ? kGatherMembers / 2 : kGatherMembers;
+    int64_t v = 0;
+    while (true) {
+        int64_t vm[M];
+        bool has[M];
+        int32_t off[M];
 #pragma unroll
-            for (int m = 0; m < M; m++) {
-                while (v < sa.ng && !block_valid(ib, g.si_half[sa.j + a.spacing * v], g.nsm,
-                                                 g.readings))
-                    v++;
-                has[m] = v < sa.ng;
-                vm[m] = v;
-                off[m] = has[m] ? (int32_t)a.slice_off[sa.j + a.spacing * v] : 0;
-                if (has[m])
-                    v++;
-            }
-            if (!has[0])
-                break;
-#pragma unroll
-            for (int m = 0; m < M; m++)
-#pragma unroll
-                for (int q = 0; q < kViewBlock; q++)
-                    accm[m][wave * kAcc + q * 64 + lane] = 0.0f;
-            for (int64_t p0 = lo; p0 < hi; p0 += 64 * kGatherUnroll) {
-                int32_t iv[kGatherUnroll], kv[kGatherUnroll];
-                float av[kGatherUnroll];
-#pragma unroll
-                for (int u = 0; u < kGatherUnroll; u++) {
-                    const int64_t p = p0 + u * 64 + lane;
-                    const bool ok = p < hi;
-                    iv[u] = ok ? g.idx[p] : 0;
-                    av[u] = ok ? gb_value<FP16>(g, p) : 0.0f;
-                    kv[u] = ok ? (int32_t)g.key[p] : -1;
-                }
-                float ev[M][kGatherUnroll], mv[M][kGatherUnroll];
-#pragma unroll
-                for (int m = 0; m < M; m++)
-#pragma unroll
-                    for (int u = 0; u < kGatherUnroll; u++) {
-                        // |ray index| + |slice offset| < 2^31 (checked at set-up)
-                        const int32_t r = iv[u] + off[m];
-                        const bool in = has[m] && kv[u] >= 0 && (uint32_t)r < (uint32_t)ds;
-                        mv[m][u] = in ? av[u] : 0.0f;
-                        ev[m][u] = ew[in ? r : 0];
-                    }
-#pragma unroll
-                for (int u = 0; u < kGatherUnroll; u++) {
-                    if (p0 + u * 64 >= hi)
-                        break;
-                    const uint64_t head = run_heads(kv[u], lane);
-                    float t[M];
-#pragma unroll
-                    for (int m = 0; m < M; m++)
-                        t[m] = run_sum(mv[m][u] * ev[m][u], head, lane);
-                    if (kv[u] >= 0 && run_end(head, lane)) {
-#pragma unroll
-                        for (int m = 0; m < M; m++)
-                            accm[m][wave * kAcc + kv[u]] += t[m];
-                    }
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < M; m++) {
-                float sm = 0.0f;
-#pragma unroll
-                for (int q = 0; q < kViewBlock; q++)
-                    sm += accm[m][wave * kAcc + q * 64 + lane];
-                if (own && has[m])
-                    g.part1[(vm[m] * g.nib + ib) * sa.nc + slot] = sm;
-            }
+        for (int m = 0; m < M; m++) {
+            while (v < sa.ng &&
+                   !block_valid(ib, g.si_half[sa.j + a.spacing * v], g.nsm, g.readings))
+                v++;
+            has[m] = v < sa.ng;
+            vm[m] = v;
+            off[m] = has[m] ? (int32_t)a.slice_off[sa.j + a.spacing * v] : 0;
+            if (has[m])
+                v++;
         }
-        return;
-    }
-    for (int64_t v = 0; v < sa.ng; v++) {
-        const int64_t k = sa.j + a.spacing * v;
-        if (!block_valid(ib, g.si_half[k], g.nsm, g.readings))
-            continue;
-        const int64_t off = a.slice_off[k];
+        if (!has[0])
+            break;
 #pragma unroll
-        for (int q = 0; q < kViewBlock; q++) {
-            a1[q * 64 + lane] = 0.0f;
-            if (THETA2)
-                a2[q * 64 + lane] = 0.0f;
-        }
+        for (int m = 0; m < (THETA2 ? 2 * M : M); m++)
+#pragma unroll
+            for (int q = 0; q < kViewBlock; q++)
+                accm[m][wave * kAcc + q * 64 + lane] = 0.0f;
         for (int64_t p0 = lo; p0 < hi; p0 += 64 * kGatherUnroll) {
             int32_t iv[kGatherUnroll], kv[kGatherUnroll];
-            float av[kGatherUnroll];
-            bool ok[kGatherUnroll];
+            float av[kGatherUnroll], sv[kGatherUnroll];
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; u++) {
                 const int64_t p = p0 + u * 64 + lane;
-                ok[u] = p < hi;
-                iv[u] = ok[u] ? g.idx[p] : 0;
-                av[u] = ok[u] ? gb_value<FP16>(g, p) : 0.0f;
-                kv[u] = ok[u] ? (int32_t)g.key[p] : -1;
+                const bool ok = p < hi;
+                iv[u] = ok ? g.idx[p] : 0;
+                av[u] = ok ? gb_value<FP16>(g, p) : 0.0f;
+                kv[u] = ok ? (int32_t)g.key[p] : -1;
             }
-            int64_t r[kGatherUnroll];
+            if (THETA2) {
 #pragma unroll
-            for (int u = 0; u < kGatherUnroll; u++) {
-                r[u] = (int64_t)iv[u] + off;
-                const bool in = ok[u] && r[u] >= 0 && r[u] < ds;
-                av[u] = in ? av[u] : 0.0f;
-                r[u] = in ? r[u] : 0;
+                for (int u = 0; u < kGatherUnroll; u++)
+                    sv[u] = g.ssum ? g.ssum[iv[u]] : av[u];
             }
-            float ev[kGatherUnroll], wv[kGatherUnroll], sv[kGatherUnroll];
+            float ev[M][kGatherUnroll], mv[M][kGatherUnroll], wv[M][kGatherUnroll];
 #pragma unroll
-            for (int u = 0; u < kGatherUnroll; u++) {
-                ev[u] = ew[r[u]];
-                wv[u] = (THETA2 && wt) ? wt[r[u]] : 1.0f;
-                sv[u] = (THETA2 && g.ssum) ? g.ssum[iv[u]] : av[u];
-            }
+            for (int m = 0; m < M; m++)
+#pragma unroll
+                for (int u = 0; u < kGatherUnroll; u++) {
+                    // |ray index| + |slice offset| < 2^31 (checked at set-up)
+                    const int32_t r = iv[u] + off[m];
+                    const bool in = has[m] && kv[u] >= 0 && (uint32_t)r < ds;
+                    mv[m][u] = in ? av[u] : 0.0f;
+                    ev[m][u] = ew[in ? r : 0];
+                    wv[m][u] = (THETA2 && wt) ? wt[in ? r : 0] : 1.0f;
+                }
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; u++) {
                 if (p0 + u * 64 >= hi)
                     break;
                 const uint64_t head = run_heads(kv[u], lane);
-                const float t1 = run_sum(av[u] * ev[u], head, lane);
-                const float t2 = THETA2 ? run_sum((wv[u] * av[u]) * sv[u], head, lane) : 0.0f;
-                if (ok[u] && run_end(head, lane)) {
-                    a1[kv[u]] += t1;
+                float t1[M], t2[M];
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    t1[m] = run_sum(mv[m][u] * ev[m][u], head, lane);
                     if (THETA2)
-                        a2[kv[u]] += t2;
+                        t2[m] = run_sum((wv[m][u] * mv[m][u]) * sv[u], head, lane);
+                }
+                if (kv[u] >= 0 && run_end(head, lane)) {
+#pragma unroll
+                    for (int m = 0; m < M; m++) {
+                        accm[m][wave * kAcc + kv[u]] += t1[m];
+                        if (THETA2)
+                            accm[M + m][wave * kAcc + kv[u]] += t2[m];
+                    }
                 }
             }
         }
-        float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
-        for (int q = 0; q < kViewBlock; q++) {
-            s1 += a1[q * 64 + lane];
-            if (THETA2)
-                s2 += a2[q * 64 + lane];
-        }
-        if (own) {
-            const int64_t o = (v * g.nib + ib) * sa.nc + slot;
-            g.part1[o] = s1;
-            if (THETA2)
-                g.part2[o] = s2;
+        for (int m = 0; m < M; m++) {
+            float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+            for (int q = 0; q < kViewBlock; q++) {
+                s1 += accm[m][wave * kAcc + q * 64 + lane];
+                if (THETA2)
+                    s2 += accm[M + m][wave * kAcc + q * 64 + lane];
+            }
+            if (own && has[m]) {
+                const int64_t o = (vm[m] * g.nib + ib) * sa.nc + slot;
+                g.part1[o] = s1;
+                if (THETA2)
+                    g.part2[o] = s2;
+            }
         }
     }
 }
