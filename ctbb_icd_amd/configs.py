@@ -7,6 +7,8 @@ the reference's key set.
 
   micro            8x8x3 volume, 4 x 16 detector: small enough for dense checks
   tiny, tiny_ffs   test geometries the CPU oracle runs in well under a second
+  tiny_zffs,       tiny_ffs with the z flying focal spot (16 detector rows,
+  tiny_zphiffs     8 per projection), and with both spots
   config0          16 x 368 detector, 576 views/turn, 128x128x32 volume,
                    4 rotations (2 fail the reference's own z-range check),
                    quadratic lambda 1e3, weights = counts, I0 = 1e5
@@ -95,6 +97,10 @@ CONFIGS["micro"] = dict(
     synthetic_n_ellipsoids=2, synthetic_ellipsoid_radius=0.4,
     synthetic_axis_min=0.1, synthetic_axis_max=0.3)
 CONFIGS["config2"] = dict(CONFIGS["config1"], ssm_fp16=1, resync_every=2)
+# z flying focal spot (Nrows 16 -> 8 projection rows) and both spots at once
+CONFIGS["tiny_zffs"] = dict(CONFIGS["tiny_ffs"], Zffs=1, Phiffs=0, Nrows=16,
+                            z_end=-0.84)
+CONFIGS["tiny_zphiffs"] = dict(CONFIGS["tiny_zffs"], Phiffs=1, Readings=576)
 CONFIGS["config3"] = dict(CONFIGS["config1"], Readings=4608, nx=256, ny=256,
                           z_start=-4.0, z_end=-2.05)
 CONFIGS["config4"] = dict(CONFIGS["config1"], Readings=13824, z_start=-16.5,

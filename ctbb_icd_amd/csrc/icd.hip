@@ -295,56 +295,11 @@ __device__ __forceinline__ T run_sum(T v, uint64_t head, int lane)
     return v;
 }
 
-#ifndef CTBB_DPP_SCAN
-#define CTBB_DPP_SCAN 0
-#endif
-
-#if CTBB_DPP_SCAN
-// FP32 run sums with DPP lane shifts instead of LDS permutes.  In-row steps
-// (row_shr 1, 2, 4, 8) treat every 16-lane row start as a run start, so
-// lane l then holds the sum over [max(run start, row start), l]; then the
-// last lane of row 0 / row 2 is added to rows 1 / 3 (row_bcast:15), and lane
-// 31 to rows 2 and 3 (row_bcast:31), where the run reaches back that far.
-// Fixed order of additions: deterministic.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_shift(float v)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF,
-                                                      true));
-}
-
-template <>
-__device__ __forceinline__ float run_sum<float>(float v, uint64_t head, int lane)
-{
-    const uint64_t below = ~0ull >> (63 - lane);
-    const int start = 63 - __clzll((unsigned long long)(head & below));
-    const int rstart = max(start, lane & ~15);
-    float t = dpp_shift<0x111, 0xF>(v);
-    if (lane - 1 >= rstart) v += t;
-    t = dpp_shift<0x112, 0xF>(v);
-    if (lane - 2 >= rstart) v += t;
-    t = dpp_shift<0x114, 0xF>(v);
-    if (lane - 4 >= rstart) v += t;
-    t = dpp_shift<0x118, 0xF>(v);
-    if (lane - 8 >= rstart) v += t;
-    t = dpp_shift<0x142, 0xA>(v);
-    if ((lane & 16) && start < (lane & ~15)) v += t;
-    t = dpp_shift<0x143, 0xC>(v);
-    if (lane >= 32 && start < 32) v += t;
-    return v;
-}
-#endif
-
 // Head mask of a wavefront whose lanes hold sorted keys (invalid lanes, at
 // the tail, carry key -1) and run-end test for lane `lane`.
 __device__ __forceinline__ uint64_t run_heads(int key, int lane)
 {
-#if CTBB_DPP_SCAN
-    // wave_shr:1 -- lane l reads lane l - 1 (lane 0 keeps the old value)
-    const int prev = __builtin_amdgcn_update_dpp(key, key, 0x138, 0xF, 0xF, false);
-#else
     const int prev = __shfl_up(key, 1, 64);
-#endif
     return __ballot(lane == 0 || prev != key);
 }
 

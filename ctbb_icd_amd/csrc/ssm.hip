@@ -61,7 +61,8 @@ constexpr int kBuildBlock = 256;
 // Exact reference entry test for ray (j, k) of view V at column (ix0, iy0).
 // Returns true and the FP32 value when the reference pushes (q, value) into
 // this column.  Mirrors src/generate_system_matrix.cpp:111-184 (no FFS) and
-// src/generate_system_matrix_ffs.cpp:147-220 (in-plane FFS).
+// src/generate_system_matrix_ffs.cpp:147-220 (flying focal spots: in-plane,
+// z, or both).
 __device__ __forceinline__ bool ray_entry(const BuildArgs &a, const ViewRec &V,
                                           int j, int k, int ix0, int iy0,
                                           float &out)
@@ -166,12 +167,15 @@ __device__ __forceinline__ void ray_window(const BuildArgs &a,
         rmax = fmaxf(rmax, rho);
     }
     const float D = (float)a.D, dz = (float)a.vz;
+    // detector rows sit at heights measured from the undeflected source: a
+    // z-deflected focal spot (off2) shifts every hit row by -off2
+    const float off2 = (float)V.off2;
     float hmin = 1e30f, hmax = -1e30f;
 #pragma unroll
     for (int cc = 0; cc < 4; cc++) {
         const float z = ((cc & 1) ? dz : -dz) - s2;
         const float rho = (cc & 2) ? rmax : rmin;
-        const float ap = -z * D / rho;
+        const float ap = -z * D / rho - off2;
         hmin = fminf(hmin, ap);
         hmax = fmaxf(hmax, ap);
     }
@@ -464,19 +468,16 @@ extern "C" int ctbb_ssm_build(const ctbb_geom *g, ctbb_ssm **out)
     if (!out)
         return fail(CTBB_E_ARG, "out is null");
     *out = nullptr;
-    int ffs;
-    if (g->zffs == 0 && g->phiffs == 0)
-        ffs = 0;
-    else if (g->zffs == 0 && g->phiffs != 0)
-        ffs = 1;
-    else
-        return fail(CTBB_E_UNSUPPORTED,
-                    "z flying focal spot SSM (Zffs != 0) is not implemented");
+    const int ffs = (g->zffs != 0 || g->phiffs != 0) ? 1 : 0;
 
     const int64_t nsm = g->n_views_sm, nch = g->n_channels, nr = g->n_rows;
     const int64_t ncol = g->nx * g->ny;
     const double D = g->source_detector_distance, R = g->focal_spot_radius;
     const double da = D * R * sin(g->fan_angle_increment) / (4.0 * (D - R));
+    // src/generate_system_matrix_ffs.cpp:66 (the constant is the file's PI)
+    const double dr = D * g->coll_slicewidth /
+                      (4.0 * (D - R) * tan(g->anode_angle * 3.14159265359 / 180.0));
+    const double tz = tan(g->anode_angle);
 
     // Host tables with the reference's libm (src/generate_system_matrix.cpp
     // :85-114, src/generate_system_matrix_ffs.cpp:112-143).
@@ -492,8 +493,28 @@ extern "C" int ctbb_ssm_build(const ctbb_geom *g, ctbb_ssm **out)
         double base[3], off[3], s[3];
         for (int e = 0; e < 3; e++)
             base[e] = R * ew[e] + zs * ez[e];
-        for (int e = 0; e < 3; e++)
-            off[e] = ffs ? ((i % 2 == 0) ? da * eu[e] : (-da) * eu[e]) : 0.0;
+        // focal-spot deflection, src/generate_system_matrix_ffs.cpp:242-283:
+        // +-da along e_u (phi), +-dr along e_w with +-dr tan(anode) along
+        // e_z (z; tan() of the prm angle as given, in degrees)
+        double cr = 0.0, ca = 0.0;
+        if (g->phiffs && !g->zffs) {
+            ca = (i % 2 == 0) ? 1.0 : -1.0;
+        } else if (!g->phiffs && g->zffs) {
+            cr = (i % 2 == 0) ? -1.0 : 1.0;
+        } else if (ffs) {
+            cr = (i % 4) < 2 ? -1.0 : 1.0;
+            ca = (i % 2 == 0) ? -1.0 : 1.0;
+        }
+        for (int e = 0; e < 3; e++) {
+            double o = 0.0;
+            if (g->zffs)
+                o = (cr * dr) * ew[e];
+            if (g->phiffs)
+                o = g->zffs ? o + (ca * da) * eu[e] : (ca * da) * eu[e];
+            if (g->zffs)
+                o = o + ((cr * dr) * tz) * ez[e];
+            off[e] = ffs ? o : 0.0;
+        }
         for (int e = 0; e < 3; e++)
             s[e] = ffs ? base[e] + off[e] : base[e];
         hv[i] = ViewRec{ew[0], ew[1], eu[0], eu[1], s[0], s[1],

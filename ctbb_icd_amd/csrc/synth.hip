@@ -4,9 +4,10 @@
 // (src/setup.cpp:245-345).  This generator produces the same `raw` layout
 // (row fastest, src/setup.cpp:335-344) for the scan a prm file describes:
 // data view v has tube angle v * 2pi / vpt, source z = table_positions[v]
-// (FileType-0 conventions, src/setup.cpp:271-284, 349-356) and, with the
-// in-plane flying focal spot, the source deflected by +-da along e_u by view
-// parity (src/generate_system_matrix_ffs.cpp:250-261).  Every detector ray
+// (FileType-0 conventions, src/setup.cpp:271-284, 349-356) and, with flying
+// focal spots, the source deflected as in the matrix build (+-da along e_u,
+// +-dr along e_w and +-dr tan(anode) along e_z by view index,
+// src/generate_system_matrix_ffs.cpp:242-283).  Every detector ray
 // uses the SSM's ray model (channel angle, row height, FFS offset), so the
 // reconstruction model and the data agree up to discretisation.
 //
@@ -51,8 +52,8 @@ struct Scene {
 
 struct ScanRays {
     int64_t readings, nch, nr, vpt;
-    int ffs;
-    double R, D, da, c_ch, c_row, dgamma_nonffs, fan_inc, drow;
+    int ffs, zffs;                  // in-plane / z flying focal spot
+    double R, D, da, dr, tz, c_ch, c_row, dgamma_nonffs, fan_inc, drow;
     const double *table_pos;
     double dtheta;
     double I0;
@@ -178,16 +179,27 @@ __host__ __device__ void synth_sample(const Scene &sc, const ScanRays &r,
     const double th = view * r.dtheta;
     const double cw = cos(th), sw = sin(th);
     const double ew[3] = {cw, sw, 0.0}, eu[3] = {-sw, cw, 0.0};
+    // focal-spot deflection of the view (src/generate_system_matrix_ffs.cpp:
+    // 242-283), as in the matrix build
     double off[3] = {0.0, 0.0, 0.0};
-    if (r.ffs) {
-        const double sgn = (view % 2 == 0) ? r.da : -r.da;
-        off[0] = sgn * eu[0];
-        off[1] = sgn * eu[1];
+    if (r.zffs || r.ffs) {
+        double cr = 0.0, ca = 0.0;
+        if (r.ffs && !r.zffs) {
+            ca = (view % 2 == 0) ? 1.0 : -1.0;
+        } else if (!r.ffs && r.zffs) {
+            cr = (view % 2 == 0) ? -1.0 : 1.0;
+        } else {
+            cr = (view % 4) < 2 ? -1.0 : 1.0;
+            ca = (view % 2 == 0) ? -1.0 : 1.0;
+        }
+        const double ez[3] = {0.0, 0.0, 1.0};
+        for (int e = 0; e < 3; e++)
+            off[e] = (cr * r.dr) * ew[e] + (ca * r.da) * eu[e] + ((cr * r.dr) * r.tz) * ez[e];
     }
     const double s[3] = {r.R * ew[0] + off[0], r.R * ew[1] + off[1],
                          r.table_pos[view] + off[2]};
-    const double gam = r.ffs ? (ch - r.c_ch) * r.fan_inc
-                             : (ch - r.c_ch) * r.dgamma_nonffs;
+    const double gam = (r.ffs || r.zffs) ? (ch - r.c_ch) * r.fan_inc
+                                         : (ch - r.c_ch) * r.dgamma_nonffs;
     const double cg = cos(gam), sg = sin(gam);
     const double ap = (row - r.c_row) * r.drow;
     double d[3];
@@ -354,13 +366,13 @@ extern "C" int ctbb_synthesize(const char *prm_path, float *raw, float *weights,
     r.nr = (int64_t)rp.Nrows_projection;
     r.vpt = (int64_t)rp.num_views_per_turn;
     r.ffs = rp.Phiffs ? 1 : 0;
-    if (rp.Zffs) {
-        free_ct(&d);
-        return ctbb::fail(CTBB_E_UNSUPPORTED, "z flying focal spot data");
-    }
+    r.zffs = rp.Zffs ? 1 : 0;
     r.R = rp.focal_spot_radius;
     r.D = rp.source_detector_distance;
     r.da = r.D * r.R * sin(rp.fan_angle_increment) / (4.0 * (r.D - r.R));
+    r.dr = r.D * rp.CollSlicewidth /
+           (4.0 * (r.D - r.R) * tan(rp.anode_angle * 3.14159265359 / 180.0));
+    r.tz = tan(rp.anode_angle);
     r.c_ch = rp.center_channel_non_ffs;
     r.c_row = rp.center_row;
     r.dgamma_nonffs = rp.transaxial_detector_spacing / r.D;

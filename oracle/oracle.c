@@ -42,6 +42,37 @@ typedef struct {
     double off[3];    /* focal-spot offset (zero without FFS)                */
 } view_geom;
 
+/* Focal-spot deflection of SSM view i: generate_ffs_offset,
+ * src/generate_system_matrix_ffs.cpp:242-283 (radial = e_w, anti-radial =
+ * e_u; the z term takes tan() of the prm anode angle as given, in degrees,
+ * while dr at :66 converts it to radians -- reproduced as written). */
+static void orc_ffs_offset(const ctbb_geom *g, int i, double da, double dr,
+                           const view_geom *v, double off[3])
+{
+    double t = tan(g->anode_angle);
+    double cr, ca, cz;             /* multiples of dr e_w, da e_u, dr tan e_z */
+    if (g->phiffs && !g->zffs) {
+        cr = 0.0; ca = (i % 2 == 0) ? 1.0 : -1.0; cz = 0.0;
+    } else if (!g->phiffs && g->zffs) {
+        cr = (i % 2 == 0) ? -1.0 : 1.0; ca = 0.0; cz = cr;
+    } else {
+        int rho = i % 4;
+        cr = rho < 2 ? -1.0 : 1.0;
+        ca = (rho % 2 == 0) ? -1.0 : 1.0;
+        cz = cr;
+    }
+    for (int e = 0; e < 3; e++) {
+        double o = 0.0;
+        if (g->zffs)
+            o = (cr * dr) * v->ew[e];
+        if (g->phiffs)
+            o = g->zffs ? o + (ca * da) * v->eu[e] : (ca * da) * v->eu[e];
+        if (g->zffs)
+            o = o + ((cz * dr) * t) * v->ez[e];
+        off[e] = o;
+    }
+}
+
 static void orc_view(const ctbb_geom *g, int i, int ffs, double da,
                      view_geom *v)
 {
@@ -56,12 +87,13 @@ static void orc_view(const ctbb_geom *g, int i, int ffs, double da,
     double base[3];
     for (int e = 0; e < 3; e++)
         base[e] = g->focal_spot_radius * v->ew[e] + zs * v->ez[e];
-    for (int e = 0; e < 3; e++) {
-        if (ffs)
-            v->off[e] = (i % 2 == 0) ? da * v->eu[e] : (-da) * v->eu[e];
-        else
-            v->off[e] = 0.0;
-    }
+    double dr = g->source_detector_distance * g->coll_slicewidth
+                / (4.0 * (g->source_detector_distance - g->focal_spot_radius)
+                   * tan(g->anode_angle * ORC_PI_FFS / 180.0));
+    if (ffs)
+        orc_ffs_offset(g, i, da, dr, v, v->off);
+    else
+        v->off[0] = v->off[1] = v->off[2] = 0.0;
     for (int e = 0; e < 3; e++)
         v->s[e] = ffs ? base[e] + v->off[e] : base[e];
 }
@@ -108,13 +140,7 @@ typedef void (*emit_fn)(void *ctx, int64_t col, int32_t q, float val);
  * ascending q. */
 static int orc_ssm_rays(const ctbb_geom *g, emit_fn emit, void *ctx)
 {
-    int ffs;
-    if (g->zffs == 0 && g->phiffs == 0)
-        ffs = 0;
-    else if (g->zffs == 0 && g->phiffs != 0)
-        ffs = 1;
-    else
-        return -1; /* z flying focal spot: not restated */
+    int ffs = (g->zffs != 0 || g->phiffs != 0);
     double D = g->source_detector_distance, R = g->focal_spot_radius;
     double da = D * R * sin(g->fan_angle_increment) / (4.0 * (D - R));
     int nx = (int)g->nx, ny = (int)g->ny;

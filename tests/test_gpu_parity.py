@@ -9,7 +9,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SSM_CASES = ["micro", "tiny", "tiny_ffs", "config0"]
+SSM_CASES = ["micro", "tiny", "tiny_ffs", "tiny_zffs", "tiny_zphiffs", "config0"]
 
 
 @pytest.fixture(scope="module")
@@ -157,6 +157,7 @@ def _icd_case(gpu, ssm_pair, orc, name, n_iter, fp16=False, resync=0, tile=8,
 @pytest.mark.parametrize("name,n_iter,tile,plain", [
     ("tiny", 3, 1024, False), ("tiny", 3, 4, False), ("tiny", 3, 4, True),
     ("tiny_ffs", 3, 1024, False), ("tiny_ffs", 3, 6, False), ("tiny_ffs", 3, 2, True),
+    ("tiny_zphiffs", 2, 4, False),
     ("config0", 1, 16, False), ("config0", 1, 8, False), ("config0", 1, 1024, False)])
 def test_icd_matches_oracle_same_order(gpu, ssm_pair, orc, name, n_iter, tile, plain):
     """tile 1024: single-column batches (the reference's column order with

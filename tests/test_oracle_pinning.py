@@ -37,26 +37,40 @@ def _consts(g):
                 dch=g.transaxial_detector_spacing, drow=g.axial_detector_spacing,
                 fai=g.fan_angle_increment, dth=g.tube_angle_increment,
                 dzt=g.tube_z_increment, tdir=g.table_direction, nsm=g.n_views_sm,
-                phiffs=g.phiffs)
+                phiffs=g.phiffs, zffs=g.zffs, anode=g.anode_angle,
+                coll=g.coll_slicewidth)
 
 
 def _source(c, i):
-    """Nominal source, in-plane unit vectors and FFS offset of SSM view i."""
+    """Nominal source, in-plane unit vectors and focal-spot offset (in-plane
+    part, z part) of SSM view i.  The deflection amounts and their order over
+    the views are the reference's (src/generate_system_matrix_ffs.cpp:66-67,
+    242-283: +-da along e_u, +-dr along e_w with +-dr tan(anode) along z,
+    tan() taken of the prm angle as given); what this pins is how the matrix
+    turns a deflected source into footprints."""
     a = (i - 0.5 * c["nsm"]) * c["dth"]
     ew = np.array([np.cos(a), np.sin(a)])
     eu = np.array([-np.sin(a), np.cos(a)])
     s = c["R"] * ew
     sz = c["tdir"] * (i - 0.5 * c["nsm"]) * c["dzt"]
-    off = np.zeros(2)
-    if c["phiffs"]:
+    off, offz = np.zeros(2), 0.0
+    if c["phiffs"] or c["zffs"]:
         da = c["D"] * c["R"] * np.sin(c["fai"]) / (4.0 * (c["D"] - c["R"]))
-        off = (da if i % 2 == 0 else -da) * eu
-    return s, sz, ew, eu, off
+        dr = c["D"] * c["coll"] / (4.0 * (c["D"] - c["R"]) * np.tan(np.radians(c["anode"])))
+        if c["phiffs"] and not c["zffs"]:
+            cr, ca = 0.0, (1.0 if i % 2 == 0 else -1.0)
+        elif c["zffs"] and not c["phiffs"]:
+            cr, ca = (-1.0 if i % 2 == 0 else 1.0), 0.0
+        else:
+            cr, ca = (-1.0 if i % 4 < 2 else 1.0), (-1.0 if i % 2 == 0 else 1.0)
+        off = cr * dr * ew + ca * da * eu
+        offz = cr * dr * np.tan(c["anode"])
+    return s, sz, ew, eu, off, offz
 
 
 def _project_point(c, i, p):
     """(channel, row) of the ray of SSM view i through the point p (x, y, z)."""
-    s, sz, ew, eu, off = _source(c, i)
+    s, sz, ew, eu, off, offz = _source(c, i)
     src = s + off
     u = p[:2] - src
     # the ray from the deflected source through p meets the detector arc
@@ -68,17 +82,18 @@ def _project_point(c, i, p):
     b = src + t * u
     v = b - s          # = -D (cos g ew + sin g eu)
     gam = np.arctan2(-(v @ eu), -(v @ ew))
-    if c["phiffs"]:
+    if c["phiffs"] or c["zffs"]:
         ch = c["cch"] + gam / c["fai"]
     else:
         ch = c["cch"] + gam * c["D"] / c["dch"]
-    # z along the ray: from sz at the source to sz - z_row at the detector
+    # z along the ray: from sz + offz at the (deflected) source to sz - z_row
+    # at the detector, whose rows are placed about the nominal source
     f = np.sqrt(A) / np.linalg.norm(b - src)
-    zrow = (sz - p[2]) / f
+    zrow = (sz + offz - p[2]) / f - offz
     return ch, c["crow"] + zrow / c["drow"]
 
 
-@pytest.mark.parametrize("name", ["tiny", "tiny_ffs"])
+@pytest.mark.parametrize("name", ["tiny", "tiny_ffs", "tiny_zffs", "tiny_zphiffs"])
 def test_ssm_footprint_centroids_follow_ray_geometry(make_scan, orc, name):
     s = make_scan(name)
     g = s.geom
@@ -131,9 +146,13 @@ def _cylinder_area_fraction(g, radius, sub=8):
     return frac.T.reshape(-1)                # index ix + nx * iy
 
 
-@pytest.mark.parametrize("name", ["tiny", "tiny_ffs"])
+@pytest.mark.parametrize("name", ["tiny", "tiny_ffs", "tiny_zffs", "tiny_zphiffs"])
 def test_cylinder_projection_matches_chord_lengths(make_scan, orc, name):
-    s = make_scan(name, z_start=-3.4, z_end=-0.4, Readings=480)
+    # a slab of a few cm (the z flying focal spot doubles the beam width and
+    # with both spots the views per turn double too)
+    span = dict(tiny_zffs=dict(z_start=-3.0, z_end=-0.9, Readings=480),
+                tiny_zphiffs=dict(z_start=-3.0, z_end=-0.9, Readings=960))
+    s = make_scan(name, **span.get(name, dict(z_start=-3.4, z_end=-0.4, Readings=480)))
     g = s.geom
     c = _consts(g)
     radius, mu = 2.0, 0.2
@@ -158,18 +177,18 @@ def test_cylinder_projection_matches_chord_lengths(make_scan, orc, name):
         i = d - (int(s.slice_index[k]) - g.n_views_sm // 2)
         if i < 0 or i >= g.n_views_sm:
             continue
-        src, sz, ew, eu, off = _source(c, i)
+        src, sz, ew, eu, off, offz = _source(c, i)
         for row in (int(np.floor(c["crow"])), int(np.ceil(c["crow"]))):
             zrow = (row - c["crow"]) * c["drow"]
             # z of the ray at the isocentre, in scan coordinates
-            ziso = s.table_positions[d] - zrow * c["R"] / c["D"]
-            margin = 0.05 + radius * abs(zrow) / c["D"]
+            ziso = s.table_positions[d] + offz - (zrow + offz) * c["R"] / c["D"]
+            margin = 0.05 + radius * abs(zrow + offz) / c["D"] + 0.1 * abs(offz)
             if not (zlo + margin < ziso < zhi - margin):
                 continue
             if off_z.size and np.min(np.abs(off_z - ziso)) < g.voxel_size_z + margin:
                 continue
             for j in range(nch):
-                if c["phiffs"]:
+                if c["phiffs"] or c["zffs"]:
                     gam = (j - c["cch"]) * c["fai"]
                 else:
                     gam = (j - c["cch"]) * c["dch"] / c["D"]
@@ -178,10 +197,12 @@ def test_cylinder_projection_matches_chord_lengths(make_scan, orc, name):
                 dxy = b - o
                 # distance of the isocentre from the in-plane ray line
                 pdist = abs(dxy[0] * o[1] - dxy[1] * o[0]) / np.linalg.norm(dxy)
-                if pdist > 0.9 * radius:
+                # near the rim the voxelised disk's chord error grows (about
+                # 5 % at 0.9 radius); keep to the well-sampled interior
+                if pdist > 0.85 * radius:
                     continue
                 chord = 2 * np.sqrt(radius ** 2 - pdist ** 2)
-                length3d = np.sqrt(dxy @ dxy + zrow ** 2) / np.linalg.norm(dxy)
+                length3d = np.sqrt(dxy @ dxy + (zrow + offz) ** 2) / np.linalg.norm(dxy)
                 expect = mu * chord * length3d
                 errs.append((y[d, j, row] - expect) / expect)
                 n += 1
