@@ -74,7 +74,8 @@ int ctbb_copy_probe(int64_t bytes, double *ms);
  * Replaces generate_system_matrix (src/generate_system_matrix.h:25,
  * src/generate_system_matrix.cpp:54-209) and generate_system_matrix_ffs
  * (src/generate_system_matrix_ffs.h:25, src/generate_system_matrix_ffs.cpp:
- * 60-290, in-plane flying focal spot only).  The matrix is built on the
+ * 60-290: in-plane, z or both flying focal spots, deflections of
+ * generate_ffs_offset, :242-283).  The matrix is built on the
  * device in compressed-sparse-column form: column c = ix + nx*iy holds
  * strictly ascending ray indices q = row + n_rows*(channel + n_channels*view)
  * over SSM-relative views [0, n_views_sm), with FP32 values (explicit zeros
