@@ -2,6 +2,8 @@
 import re
 from pathlib import Path
 
+import pytest
+
 ROOT = Path(__file__).resolve().parent.parent
 
 
@@ -41,3 +43,13 @@ def test_device_entry_points_reject_null(ct):
     import ctypes
     assert ct.lib().ctbb_ssm_build(None, None) == -1
     assert ct.lib().ctbb_recon_iterate(None, 1, None) == -1
+
+
+def test_empty_slab_is_rejected(ct, tmp_path):
+    """z_start == z_end gives no slices; the matrix build refuses the
+    geometry before touching the device."""
+    from ctbb_icd_amd import configs
+    s = ct.Scan(configs.write_prm("tiny", tmp_path / "empty.yaml", z_start=-1.5, z_end=-1.5))
+    assert s.geom.nz == 0
+    with pytest.raises(ct.CtbbError, match="positive"):
+        ct.SSM.build(s.geom)

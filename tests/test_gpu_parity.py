@@ -244,3 +244,28 @@ def test_running_cost_matches_fp64_cost(gpu, ssm_pair, orc):
     print(f"[running-cost] 2 sweeps: fp64 {c64:.10e} running {c32:.10e}", flush=True)
     assert abs(c32 - c64) <= 1e-5 * abs(c64)
     r.free()
+
+
+@pytest.mark.parametrize("name,over,tile", [
+    ("micro", {}, 3),                                   # tiles cut the image raggedly
+    ("tiny", dict(z_start=-1.5, z_end=-1.38), 4),       # a single slice
+    ("tiny", dict(nx=23, ny=21), 4),                    # odd, non-square image
+])
+def test_icd_edge_shapes_match_oracle(gpu, make_scan, orc, name, over, tile):
+    s = make_scan(name, **over)
+    g = s.geom
+    raw, w = s.synthesize(use_gpu=False)
+    m = gpu.SSM.build(g)
+    host = orc.ssm_build(g)
+    sp = m.group_spacing(s.slice_index)
+    r = gpu.Recon(m, raw, w, s.slice_index, s.opts(tile=tile, group_spacing=sp))
+    st = r.iterate(2)
+    xg = r.volume().astype(np.float64)
+    xo, _, nupd = orc.icd(g, host, raw, w, s.slice_index, s.run.penalty, s.run.lambda_,
+                          s.run.delta, sp, 2, tile=tile, surrogate=True)
+    print(f"[edge] {name} {over} tile {tile}: {g.nx}x{g.ny}x{g.nz}, spacing {sp}, "
+          f"{st.voxel_updates} updates", flush=True)
+    assert st.voxel_updates == nupd
+    rms = np.sqrt(np.mean((xg - xo) ** 2)) / max(np.sqrt(np.mean(xo ** 2)), 1e-30)
+    assert rms <= 1e-4, rms
+    r.free()
