@@ -1177,6 +1177,8 @@ __global__ void rm_group_kernel(const int64_t *rptr, int64_t ngroups, int64_t nr
 // shuffles and adds each run into an FP64 LDS tile.  Fixed order, no
 // atomics.  MODE 0: yhat = A x.  MODE 1: e = (float)(y - A x) (if e) and
 // one partial of w (y - A x)^2 per workgroup.  LDS: kScatterCh * nr doubles.
+constexpr int kProjUnroll = 4;
+
 template <typename X, int MODE>
 __global__ __launch_bounds__(kSweepBlock) void project_gather_kernel(
     const int64_t *vc_ptr, const uint32_t *rent, const float *rval, const X *x,
@@ -1203,16 +1205,32 @@ __global__ __launch_bounds__(kSweepBlock) void project_gather_kernel(
         for (int64_t c = c0 + wave; c < c1; c += kWaves) {
             const int64_t lo = vc_ptr[i * nch + c], hi = vc_ptr[i * nch + c + 1];
             double *accc = pacc + (c - c0) * nr;
-            for (int64_t p0 = lo; p0 < hi; p0 += 64) {
-                const int64_t p = p0 + lane;
-                const bool ok = p < hi;
-                const uint32_t e0 = ok ? rent[p] : 0u;
-                const int row = ok ? (int)(e0 >> kRmColBits) : -1;
-                double d = ok ? (double)xk[e0 & kRmColMask] * (double)rval[p] : 0.0;
-                const uint64_t head = run_heads(row, lane);
-                d = run_sum(d, head, lane);
-                if (ok && run_end(head, lane))
-                    accc[row] += d;
+            // kProjUnroll chunks of 64 entries in flight, applied in order
+            for (int64_t p0 = lo; p0 < hi; p0 += 64 * kProjUnroll) {
+                uint32_t e0[kProjUnroll];
+                float av[kProjUnroll];
+                bool ok[kProjUnroll];
+#pragma unroll
+                for (int u = 0; u < kProjUnroll; u++) {
+                    const int64_t p = p0 + u * 64 + lane;
+                    ok[u] = p < hi;
+                    e0[u] = ok[u] ? rent[p] : 0u;
+                    av[u] = ok[u] ? rval[p] : 0.0f;
+                }
+                double d[kProjUnroll];
+#pragma unroll
+                for (int u = 0; u < kProjUnroll; u++)
+                    d[u] = ok[u] ? (double)xk[e0[u] & kRmColMask] * (double)av[u] : 0.0;
+#pragma unroll
+                for (int u = 0; u < kProjUnroll; u++) {
+                    if (p0 + u * 64 >= hi)
+                        break;
+                    const int row = ok[u] ? (int)(e0[u] >> kRmColBits) : -1;
+                    const uint64_t head = run_heads(row, lane);
+                    const double t = run_sum(d[u], head, lane);
+                    if (ok[u] && run_end(head, lane))
+                        accc[row] += t;
+                }
             }
         }
     }

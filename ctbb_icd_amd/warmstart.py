@@ -178,19 +178,30 @@ def warm_run(scan: Scan, ssm: SSM, raw, w, net: UNet, n_early: int, target: floa
     r = Recon(ssm, raw, w, scan.slice_index, scan.opts())
     try:
         t0 = time.perf_counter()
+        marks = [t0]
         r.iterate(n_early)
+        marks.append(time.perf_counter())
         x_plain = r.volume()
-        x_warm = to_rotating(scan, predict_fixed(net, to_fixed(scan, x_plain)))
+        fixed = to_fixed(scan, x_plain)
+        marks.append(time.perf_counter())
+        pred = predict_fixed(net, fixed)
+        marks.append(time.perf_counter())
+        x_warm = to_rotating(scan, pred)
+        marks.append(time.perf_counter())
         c_plain = r.cost_running()
         r.set_volume(x_warm)
         c_warm = r.cost_running()
+        marks.append(time.perf_counter())
         run.accepted = bool(c_warm <= c_plain)
         if not run.accepted:
             r.set_volume(x_plain)
         run.iterations = n_early
         spent = time.perf_counter() - t0
+        ph = np.diff(marks)
         log(f"[warmstart] start cost plain {c_plain:.8e} warm {c_warm:.8e} -> "
-            f"{'accepted' if run.accepted else 'rejected'} ({spent:.2f} s)")
+            f"{'accepted' if run.accepted else 'rejected'} ({spent:.2f} s: early "
+            f"{ph[0]:.2f}, to fixed {ph[1]:.2f}, network {ph[2]:.2f}, to rotating "
+            f"{ph[3]:.2f}, projection + checks {ph[4]:.2f})")
         run.costs = [(cost_fn or (lambda rr: rr.cost()))(r)]
         if run.costs[0] <= target:
             run.reached = True
