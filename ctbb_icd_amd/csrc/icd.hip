@@ -349,12 +349,14 @@ struct StepArgs {
 #endif
 constexpr int kViewBlock = CTBB_VIEW_BLOCK;
 constexpr int kGatherUnroll = CTBB_GATHER_UNROLL;
+#ifndef CTBB_GATHER_MEMBERS_FIRST
+#define CTBB_GATHER_MEMBERS_FIRST 2
+#endif
 // Members of the slice group a wavefront takes per pass over its entries in
-// the later sweeps; the first sweep, which also sums theta2, takes half as
-// many (even, >= 2)
+// the later sweeps and in the first sweep (which also sums theta2)
 constexpr int kGatherMembers = CTBB_GATHER_MEMBERS;
-static_assert(kGatherMembers >= 2 && kGatherMembers % 2 == 0,
-              "an even number of LDS accumulators is needed");
+constexpr int kGatherMembersFirst = CTBB_GATHER_MEMBERS_FIRST;
+static_assert(kGatherMembers >= 1 && kGatherMembersFirst >= 1, "members per pass");
 
 struct GatherArgs {
     const int32_t *idx;       // view-blocked copy (see above)
@@ -398,7 +400,8 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                                                                  GatherArgs g)
 {
     constexpr int kAcc = 64 * kViewBlock;
-    __shared__ float accm[kGatherMembers][kWaves * kAcc];
+    constexpr int M = THETA2 ? kGatherMembersFirst : kGatherMembers;
+    __shared__ float accm[THETA2 ? 2 * M : M][kWaves * kAcc];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t nwg = g.nib * g.nsc;
     const int64_t per_xcd = (nwg + kXcd - 1) / kXcd;
@@ -418,10 +421,9 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
     const float *__restrict__ wt = a.w;
     const uint32_t ds = (uint32_t)a.data_size;
     // M members of the slice group per pass over the chunk's entries, each
-    // with its own LDS accumulators (the first sweep also sums theta2 and
-    // takes half as many members), so every entry is loaded once for all of
-    // them and M times as many gathers are in flight
-    constexpr int M = THETA2 ? kGatherMembers / 2 : kGatherMembers;
+    // with its own LDS accumulators (two on the first sweep, which also sums
+    // theta2), so every entry is loaded once for all of them and M times as
+    // many gathers are in flight
     int64_t v = 0;
     while (true) {
         int64_t vm[M];
