@@ -624,14 +624,16 @@ constexpr int kScatterCh = CTBB_SCATTER_CH;
 constexpr int kScatterViews = CTBB_SCATTER_VIEWS;
 constexpr int kWaveCh = kScatterCh / kWaves;
 constexpr int kItemUnroll = CTBB_ITEM_UNROLL;
-constexpr uint32_t kSlotBits = 14, kRowBits = 8;
+// Transposed entry: slot | lkey << 14 | (ch / kScatterCh) << 26, where
+// lkey = (ch % kScatterCh) * nr + row is the sample's place in the scatter
+// tile of its channel group (needs kScatterCh * nr <= 4096).
+constexpr uint32_t kSlotBits = 14, kLkeyBits = 12;
 
 __device__ __forceinline__ uint32_t bt_slot(uint32_t p) { return p & ((1u << kSlotBits) - 1); }
-__device__ __forceinline__ uint32_t bt_row(uint32_t p) { return (p >> kSlotBits) & ((1u << kRowBits) - 1); }
-__device__ __forceinline__ uint32_t bt_ch(uint32_t p) { return p >> (kSlotBits + kRowBits); }
+__device__ __forceinline__ uint32_t bt_lkey(uint32_t p) { return (p >> kSlotBits) & ((1u << kLkeyBits) - 1); }
 
 struct BtArgs {
-    const uint32_t *ent;     // packed slot | row << 14 | ch << 22
+    const uint32_t *ent;     // packed slot | lkey << 14 | channel group << 26
     const float *val;        // entry values (FP32)
     const int32_t *vc_idx;   // [nbatch][nsm*nch+1] segment starts per (view, ch)
     const int64_t *ent_off;  // [nbatch] first entry of each batch
@@ -641,11 +643,10 @@ struct BtArgs {
 
 // One 64-entry chunk of an item: reduce runs of equal samples and add them
 // into the view's tile (acc: rows of nr samples per channel from c0).
-__device__ __forceinline__ void scatter_chunk(uint32_t e0, float d, bool ok,
-                                              int c0, int nr, int lane,
+__device__ __forceinline__ void scatter_chunk(uint32_t e0, float d, bool ok, int lane,
                                               float *acc)
 {
-    const int key = ok ? ((int)bt_ch(e0) - c0) * nr + (int)bt_row(e0) : -1;
+    const int key = ok ? (int)bt_lkey(e0) : -1;
     const uint64_t head = run_heads(key, lane);
     d = run_sum(d, head, lane);
     if (ok && run_end(head, lane))
@@ -763,8 +764,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
             for (int u = 0; u < kItemUnroll; u++) {
                 if (it[u] < 0)
                     continue;
-                scatter_chunk(ev[u], dv[u], ok[u], (int)c0, nr, lane,
-                              acc + (it[u] / kItemLanes) * tile);
+                scatter_chunk(ev[u], dv[u], ok[u], lane, acc + (it[u] / kItemLanes) * tile);
             }
         }
     }
@@ -866,7 +866,8 @@ __global__ void bt_fill_kernel(const int64_t *col_ptr, const int32_t *idx,
             const uint32_t ch = (uint32_t)(key % nch);
             const int64_t pos = ent_off[b] + vc_idx[b * nvc + key] +
                                 atomicAdd(&cursor[b * nvc + key], 1u);
-            ent[pos] = slot | (row << kSlotBits) | (ch << (kSlotBits + kRowBits));
+            const uint32_t lkey = (ch % kScatterCh) * (uint32_t)nr + row;
+            ent[pos] = slot | (lkey << kSlotBits) | ((ch / kScatterCh) << (kSlotBits + kLkeyBits));
             bval[pos] = val16 ? __half2float(val16[p]) * col_scale[col] : val[p];
         }
     }
@@ -1621,7 +1622,8 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
 // batch does not touch are not written and never read.
 __global__ void batch_ray_sum_kernel(const uint32_t *bent, const float *bval,
                                      const int32_t *vc_idx, const int64_t *ent_off,
-                                     int64_t b, int64_t nvc, int64_t nr, float *s)
+                                     int64_t b, int64_t nvc, int64_t nch, int64_t nr,
+                                     float *s)
 {
     const uint32_t *ent = bent + ent_off[b];
     const float *val = bval + ent_off[b];
@@ -1630,10 +1632,11 @@ __global__ void batch_ray_sum_kernel(const uint32_t *bent, const float *bval,
     for (int64_t seg = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; seg < nseg;
          seg += (int64_t)gridDim.x * blockDim.x) {
         const int64_t lo = vci[seg], hi = vci[seg + 1];
+        const int64_t rbase = (seg % nch) % kScatterCh * nr;   // lkey of row 0
         int64_t cur = -1;
         float acc = 0.0f;
         for (int64_t p = lo; p < hi; p++) {
-            const int64_t row = bt_row(ent[p]);
+            const int64_t row = (int64_t)bt_lkey(ent[p]) - rbase;
             if (row != cur) {
                 if (cur >= 0)
                     s[seg * nr + cur] = acc;
@@ -1912,6 +1915,7 @@ static int build_transposed_batches(ctbb_recon *r, const std::vector<int32_t> &c
     const ctbb_geom *g = &r->g;
     const int64_t nbatch = r->nbatch, ncol = g->nx * g->ny;
     const int64_t nvc = g->n_views_sm * g->n_channels + 1;
+    static_assert(kScatterCh * 128 <= (1 << kLkeyBits), "tile-local sample key width");
     if (g->n_rows > 128 || g->n_channels > 1024)
         return fail(CTBB_E_UNSUPPORTED, "transposed batches need n_rows <= 128 "
                                         "and n_channels <= 1024");
@@ -2437,7 +2441,7 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                 hipLaunchKernelGGL(batch_ray_sum_kernel,
                                    dim3((unsigned)std::min<int64_t>(8192, (nvc + 255) / 256)),
                                    dim3(256), 0, r->stream, r->bt_ent, r->bt_val, r->bt_vc,
-                                   r->bt_off, b, nvc, r->g.n_rows, r->ssum);
+                                   r->bt_off, b, nvc, r->g.n_channels, r->g.n_rows, r->ssum);
             }
             for (int64_t j = 0; j < r->spacing && j < r->g.nz; j++) {
                 StepArgs sa;
