@@ -109,10 +109,11 @@ struct ctbb_recon {
     int64_t *batch_ptr = nullptr;  // [nbatch+1] offsets into cols
     float *ew = nullptr;        // weighted error w e (weighted recons)
     float *du = nullptr;        // per-step voxel updates
-    double *th2 = nullptr;      // theta2 per voxel [nvox], iteration-invariant
-    bool th2_valid = false;     // th2 filled (by the first sweep)
-    // view-blocked batch copy of the matrix (gather): entries of batch b
-    // ordered by (view block ib, column slot), view blocks of kViewBlock views
+    double *th2 = nullptr;      // theta2 (batch surrogate curvature) per voxel
+    bool th2_valid = false;     //   [nvox], iteration-invariant, first sweep
+    // view-blocked batch copy of the matrix (gather): per batch b and view
+    // block ib (kViewBlock views) the columns ranked by detector position,
+    // in chunks of 64 ranks; a chunk's entries ordered (view in block, rank)
     int32_t *gb_idx = nullptr;    // [fov entries] ray index (as in the SSM)
     float *gb_val = nullptr;      // [fov entries] FP32 value (FP32 matrices)
     __half *gb_val16 = nullptr;   // [fov entries] FP16 value (FP16 matrices)
@@ -133,7 +134,7 @@ struct ctbb_recon {
     std::vector<int64_t> slice_off_h;   // [nz] sample offsets (host copy)
     std::vector<int64_t> h_si_half;     // [nz] central view - n_views_sm/2
     // transposed batches (sample-driven scatter)
-    uint32_t *bt_ent = nullptr;   // packed slot | row << 14 | channel << 22
+    uint32_t *bt_ent = nullptr;   // packed slot | lkey << 14 | channel group << 26
     float *bt_val = nullptr;
     int32_t *bt_vc = nullptr;     // [nbatch][n_views_sm*n_channels+1]
     int64_t *bt_off = nullptr;    // [nbatch]
