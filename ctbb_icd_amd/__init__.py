@@ -120,6 +120,7 @@ def lib() -> C.CDLL:
         "ctbb_ssm_read_file": ([C.POINTER(Geom), C.c_char_p, C.POINTER(P)], C.c_int),
         "ctbb_ssm_group_spacing": ([P, P, _i64, _f64, P], C.c_int),
         "ctbb_forward_project": ([P, P, P, P], C.c_int),
+        "ctbb_fbp": ([P, P, P, P], C.c_int),
         "ctbb_recon_create": ([P, P, P, P, C.POINTER(IcdOpts), C.POINTER(P)], C.c_int),
         "ctbb_recon_free": ([P], C.c_int),
         "ctbb_recon_set_volume": ([P, P], C.c_int),
@@ -307,6 +308,15 @@ class SSM:
         y = np.zeros(self.geom.data_size, np.float64)
         check(lib().ctbb_forward_project(self.h, _p(si), _p(x), _p(y)))
         return y
+
+    def fbp(self, raw: np.ndarray, slice_index: np.ndarray) -> np.ndarray:
+        """Filtered backprojection start image (rotating frame, FP32)."""
+        si = np.ascontiguousarray(slice_index, np.int64)
+        raw = np.ascontiguousarray(raw, np.float32)
+        g = self.geom
+        x = np.zeros(g.nvox, np.float32)
+        check(lib().ctbb_fbp(self.h, _p(raw), _p(si), _p(x)))
+        return x
 
     def free(self) -> None:
         if self.h:

@@ -144,10 +144,11 @@ void initialize_recon_volume(const struct recon_params *rp, struct ct_data *data
             exit(1);
         }
     } else if (rp->wfbp_initialize) {
-        std::cout << "wFBP initialisation calls an external FBP program that is not "
-                     "part of this build. Exiting"
+        // the external wFBP program is replaced by a GPU FBP through the
+        // system matrix; it needs the matrix, so icd_iteration forms it
+        std::cout << "Initial volume: GPU filtered backprojection (formed with the "
+                     "system matrix before the first iteration)"
                   << std::endl;
-        exit(1);
     }
     // otherwise: zeros (allocation default)
 }
@@ -195,6 +196,11 @@ void icd_iteration(const struct recon_params *rp, struct ct_data *data)
 {
     ctbb_geom g = geom_of(rp);
     std::vector<int64_t> si = slice_index_of(rp, data);
+    if (rp->wfbp_initialize && rp->initial_recon_path.compare("") == 0) {
+        if (ctbb_fbp(data->ssm, data->raw, si.data(), data->recon_volume) != CTBB_OK)
+            die(ctbb_last_error());
+        std::cout << "Initialized from the GPU FBP image" << std::endl;
+    }
     if (rp->ssm_fp16 && ctbb_ssm_to_fp16(data->ssm) != CTBB_OK)
         die("FP16 conversion failed");
     ctbb_icd_opts o;

@@ -132,6 +132,18 @@ int ctbb_ssm_group_spacing(const ctbb_ssm *m, const int64_t *slice_index,
 int ctbb_forward_project(const ctbb_ssm *m, const int64_t *slice_index,
                          const double *x, double *yhat);
 
+/* ---- filtered backprojection start image -------------------------------
+ * Stands in for the external wFBP program of initialize_recon_volume
+ * (src/initialize_recon_volume.cpp:82-178, wfbp_initialize != 0) followed
+ * by rotate_slices_fixed2rotating: every rotated slice is reconstructed in
+ * its own frame by an equiangular fan-beam FBP (ramp filter along channels,
+ * cos(gamma) and FDK cone pre-weights) backprojected through the matrix
+ * (footprint-weighted per view, 1/L^2 weighted, redundant views averaged).
+ * raw: [readings*n_channels*n_rows] (reference layout); slice_index[nz];
+ * volume: host FP32 [nx*ny*nz], rotating frame, zero outside the FOV. */
+int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slice_index,
+             float *volume);
+
 /* ---- ICD reconstruction context ----------------------------------------
  * Replaces icd_iteration (src/icd_iteration.h:23, src/icd_iteration.cpp:
  * 42-286) with the penalties of src/penalties.cpp:39-154 fused in.

@@ -1,0 +1,98 @@
+"""numpy restatement of the GPU filtered backprojection start image
+(ctbb_fbp, include/ctbb_icd.h).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the GPU
+kernels.  FP64 throughout.  No reference counterpart: the reference starts
+from an external wFBP program (src/initialize_recon_volume.cpp:82-178);
+this is an equiangular fan-beam FBP (Kak & Slaney, Principles of
+Computerized Tomographic Imaging, ch. 3.4.2) per rotated slice, with an FDK
+cone weight, backprojected through the stored system matrix:
+
+  R(view, ch, row) = y * R_src * cos(gamma_ch) * D / sqrt(D^2 + ap_row^2)
+  Q(view, ch, row) = alpha * sum_k R(view, k, row) g(ch - k),
+      g(0) = 1 / (8 alpha^2), g(n even) = 0,
+      g(n odd) = -1 / (2 pi^2 sin^2(n alpha))
+  x(c, k) = (2 pi / N_c) * sum over SSM views v of
+      (sum_e a_e Q[ray_e + off_k]) / (sum_e a_e) / L_v(c)^2
+  (entries e of column c in view v whose sample lies in the sinogram; N_c:
+  the number of views with such entries, which averages redundant coverage;
+  L_v(c): in-plane distance from the view's nominal source to the column).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def channel_angles(g):
+    ffs = bool(g.zffs or g.phiffs)
+    j = np.arange(g.n_channels, dtype=np.float64)
+    if ffs:
+        return (j - g.center_channel_non_ffs) * g.fan_angle_increment, g.fan_angle_increment
+    a = g.transaxial_detector_spacing / g.source_detector_distance
+    return (j - g.center_channel_non_ffs) * a, a
+
+
+def filter_sinogram(g, raw):
+    """Q [readings, n_channels, n_rows] from raw in the reference layout."""
+    nv, nch, nr = g.readings, g.n_channels, g.n_rows
+    p = np.asarray(raw, np.float64).reshape(nv, nch, nr)
+    gam, alpha = channel_angles(g)
+    ap = (np.arange(nr) - g.center_row) * g.axial_detector_spacing
+    D = g.source_detector_distance
+    w = (g.focal_spot_radius * np.cos(gam))[:, None] * (D / np.sqrt(D * D + ap * ap))[None, :]
+    R = p * w[None, :, :]
+    n = np.arange(-(nch - 1), nch)
+    h = np.zeros(n.size)
+    h[n == 0] = 1.0 / (8.0 * alpha * alpha)
+    odd = (n % 2) != 0
+    h[odd] = -1.0 / (2.0 * np.pi ** 2 * np.sin(n[odd] * alpha) ** 2)
+    Q = np.zeros_like(R)
+    for j in range(nch):
+        # Q[j] = alpha * sum_k R[k] h[j - k]
+        Q[:, j, :] = alpha * np.tensordot(R, h[(j - np.arange(nch)) + (nch - 1)], axes=([1], [0]))
+    return Q
+
+
+def fbp(g, ssm, raw, slice_index):
+    """Rotating-frame FBP volume [nz * ny * nx] (index c + nx*ny*k)."""
+    cp, idx, val = ssm
+    Q = filter_sinogram(g, raw).reshape(-1)
+    ds = Q.size
+    nsm, nch, nr = g.n_views_sm, g.n_channels, g.n_rows
+    nffs = (1 << int(g.zffs)) * (1 << int(g.phiffs))
+    offs = []
+    for k in range(g.nz):
+        ci = int(slice_index[k])
+        if g.zffs or g.phiffs:
+            ci -= ci % nffs
+        offs.append((ci - nsm // 2) * nch * nr)
+    ncol = g.nx * g.ny
+    x = np.zeros(g.nz * ncol)
+    fov = (g.acquisition_fov / 2.0) ** 2
+    R = g.focal_spot_radius
+    for c in range(ncol):
+        ix, iy = c % g.nx, c // g.nx
+        px = (ix - g.center_voxel_x) * g.voxel_size_x
+        py = (iy - g.center_voxel_y) * g.voxel_size_y
+        if not (px * px + py * py < fov):
+            continue
+        q = idx[cp[c]:cp[c + 1]].astype(np.int64)
+        a = val[cp[c]:cp[c + 1]].astype(np.float64)
+        if q.size == 0:
+            continue
+        view = q // (nch * nr)
+        th = (view - 0.5 * nsm) * g.tube_angle_increment
+        L2 = (R * np.cos(th) - px) ** 2 + (R * np.sin(th) - py) ** 2
+        starts = np.flatnonzero(np.r_[True, view[1:] != view[:-1]])
+        for k in range(g.nz):
+            r = q + offs[k]
+            ok = (r >= 0) & (r < ds)
+            aq = np.where(ok, a * Q[np.clip(r, 0, ds - 1)], 0.0)
+            aa = np.where(ok, a, 0.0)
+            sq = np.add.reduceat(aq, starts)
+            sa = np.add.reduceat(aa, starts)
+            l2 = L2[starts]
+            m = sa > 0
+            if m.any():
+                x[c + ncol * k] = np.sum(sq[m] / sa[m] / l2[m]) * (2.0 * np.pi / m.sum())
+    return x

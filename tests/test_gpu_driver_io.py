@@ -117,3 +117,23 @@ def test_driver_reads_binary_frames(gpu, ct, tmp_path):
     r.iterate(1)
     fixed = ct.rotate_slices(g, s.tube_angles, s.slice_index, r.volume(), 1)
     assert np.array_equal(vol, fixed)
+
+
+def test_driver_fbp_start(gpu, ct, tmp_path):
+    """wfbp_initialize: 1 starts from the GPU FBP image (the external wFBP
+    program of the reference is replaced); the run completes and differs
+    from the zero start."""
+    from ctbb_icd_amd import configs
+    outs = []
+    for wfbp in (0, 1):
+        out = tmp_path / f"out_fbp{wfbp}"
+        (out / "reconstructions").mkdir(parents=True)
+        prm = configs.write_prm("tiny_ffs", tmp_path / f"fbp{wfbp}.yaml", output_dir=str(out),
+                                num_iterations=1, wfbp_initialize=wfbp)
+        p = subprocess.run([str(ROOT / "ctbb_icd_amd" / "ctbb_icd"), str(prm)],
+                           capture_output=True, text=True, timeout=120)
+        assert p.returncode == 0, p.stdout + p.stderr
+        if wfbp:
+            assert "Initialized from the GPU FBP image" in p.stdout
+        outs.append(np.fromfile(out / "reconstruction.img", np.float32))
+    assert not np.array_equal(outs[0], outs[1])
