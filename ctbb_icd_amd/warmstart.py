@@ -213,6 +213,29 @@ def warm_run(scan: Scan, ssm: SSM, raw, w, net: UNet, n_early: int, target: floa
         r.free()
 
 
+def fbp_run(scan: Scan, ssm: SSM, raw, w, target: float, max_iter: int, log=print,
+            cost_fn=None) -> Run:
+    """ICD from the GPU filtered backprojection start (ctbb_fbp, clamped at
+    0); the time includes the FBP and the projection of the start image."""
+    run = Run("fbp")
+    r = Recon(ssm, raw, w, scan.slice_index, scan.opts())
+    try:
+        t0 = time.perf_counter()
+        r.set_volume(np.maximum(ssm.fbp(raw, scan.slice_index), 0.0).astype(np.float32))
+        spent = time.perf_counter() - t0
+        run.accepted = True
+        run.costs = [(cost_fn or (lambda rr: rr.cost()))(r)]
+        log(f"[warmstart] FBP start cost {run.costs[0]:.8e} ({spent:.2f} s)")
+        if run.costs[0] <= target:
+            run.reached = True
+            run.seconds = spent
+            return run
+        return run_to_target(r, target, max_iter, time.perf_counter() - spent, run, log,
+                             cost_fn)
+    finally:
+        r.free()
+
+
 def _main():
     import argparse
     import json
@@ -236,6 +259,8 @@ def _main():
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--fbp", action="store_true",
+                    help="also time ICD from the GPU FBP start (same matrix as the warm run)")
     ap.add_argument("--fp16-warm", action="store_true",
                     help="warm runs on the FP16 SSM, judged with the FP32 model's cost")
     ap.add_argument("--workdir", default="/tmp/ctbb_warmstart")
@@ -325,7 +350,12 @@ def _main():
                 def cost_fn(rr, raw=raw, w=w, scan=scan, opts=opts):
                     return model_cost(ssm, raw, w, scan.slice_index, opts, rr.volume())
             warm = warm_run(scan, ssm_w, raw, w, net, a.early, target, a.ref_its, log, cost_fn)
-            results.append(dict(seed=s, reference_cost=final, target=target,
+            fbp = None
+            if a.fbp:
+                fr = fbp_run(scan, ssm_w, raw, w, target, a.ref_its, log, cost_fn)
+                fbp = dict(iterations=fr.iterations, seconds=fr.seconds, reached=fr.reached,
+                           costs=fr.costs)
+            results.append(dict(seed=s, reference_cost=final, target=target, fbp=fbp,
                                 plain_iterations=n_plain, plain_seconds=plain["seconds"],
                                 plain_seconds_per_iteration=float(np.mean(secs)),
                                 warm_iterations=warm.iterations, warm_seconds=warm.seconds,
