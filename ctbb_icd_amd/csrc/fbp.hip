@@ -55,6 +55,18 @@ __global__ __launch_bounds__(kFbpBlock) void fbp_filter_kernel(
     }
 }
 
+constexpr int kFbpUnroll = 8;
+
+// Squared in-plane distance of a column at (px, py) from the nominal source
+// of SSM view v.
+__device__ __forceinline__ double fbp_l2(int64_t v, int64_t nsm, double dtheta, double R,
+                                         double px, double py)
+{
+    const double th = ((double)v - 0.5 * (double)nsm) * dtheta;
+    const double dx = R * cos(th) - px, dy = R * sin(th) - py;
+    return dx * dx + dy * dy;
+}
+
 // One wavefront per FOV column; lane l takes slices l, l + 64, ...  The
 // column's entries are read once per 64 slices (every lane reads the same
 // entry); each lane gathers Q at its slice's sample.  Sums in FP64.
@@ -80,29 +92,50 @@ __global__ __launch_bounds__(kFbpBlock) void fbp_backproject_kernel(
             const int64_t k = k0 + lane;
             const bool own = k < nz;
             const int64_t off = own ? slice_off[k] : 0;
-            double acc = 0.0;
-            int64_t nv = 0;
-            for (int64_t v = 0; v < nsm; v++) {
-                const int64_t e0 = vo[v];
-                const int64_t e1 = v + 1 < nsm ? (int64_t)vo[v + 1] : len;
-                if (e1 <= e0)
-                    continue;
-                double sa = 0.0, sq = 0.0;
-                for (int64_t e = e0; e < e1; e++) {
-                    const double a = FP16 ? (double)__half2float(val16[b + e]) * (double)sc
-                                          : (double)val[b + e];
-                    const int64_t r = (int64_t)idx[b + e] + off;
-                    if (own && r >= 0 && r < ds) {
-                        sa += a;
-                        sq += a * (double)q[r];
+            // the column's entries run view by view; kFbpUnroll of them are
+            // loaded and gathered at once, then summed in entry order, a
+            // view's footprint sum closing where its entries end
+            double acc = 0.0, sa = 0.0, sq = 0.0;
+            int64_t nv = 0, v = 0;
+            int64_t e1 = nsm > 1 ? (int64_t)vo[1] : len;
+            for (int64_t e0 = 0; e0 < len; e0 += kFbpUnroll) {
+                double av[kFbpUnroll], qv[kFbpUnroll];
+                bool ok[kFbpUnroll];
+#pragma unroll
+                for (int u = 0; u < kFbpUnroll; u++) {
+                    const int64_t e = e0 + u;
+                    const bool in = e < len;
+                    const int64_t r = in ? (int64_t)idx[b + e] + off : -1;
+                    ok[u] = own && in && r >= 0 && r < ds;
+                    av[u] = in ? (FP16 ? (double)__half2float(val16[b + e]) * (double)sc
+                                       : (double)val[b + e])
+                               : 0.0;
+                    qv[u] = ok[u] ? (double)q[r] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < kFbpUnroll; u++) {
+                    const int64_t e = e0 + u;
+                    if (e >= len)
+                        break;
+                    while (e >= e1) {     // close view v (and any empty ones)
+                        if (sa > 0.0) {
+                            acc += sq / sa / fbp_l2(v, nsm, dtheta, R, px, py);
+                            nv++;
+                        }
+                        sa = 0.0;
+                        sq = 0.0;
+                        v++;
+                        e1 = v + 1 < nsm ? (int64_t)vo[v + 1] : len;
+                    }
+                    if (ok[u]) {
+                        sa += av[u];
+                        sq += av[u] * qv[u];
                     }
                 }
-                if (sa > 0.0) {
-                    const double th = ((double)v - 0.5 * (double)nsm) * dtheta;
-                    const double dx = R * cos(th) - px, dy = R * sin(th) - py;
-                    acc += sq / sa / (dx * dx + dy * dy);
-                    nv++;
-                }
+            }
+            if (sa > 0.0) {
+                acc += sq / sa / fbp_l2(v, nsm, dtheta, R, px, py);
+                nv++;
             }
             if (own)
                 x[col + nxy * k] = nv ? (float)(acc * (2.0 * M_PI / (double)nv)) : 0.0f;
