@@ -33,25 +33,26 @@ namespace {
 constexpr int kFbpBlock = 256;
 
 // One workgroup per (view, row): the channel line, pre-weighted, in LDS;
-// thread j forms Q[j] in FP64 (the ramp kernel sums terms of both signs).
+// thread j forms Q[j] (FP32 products and sums: FP64 would cost 2.4e11 FP64
+// FMAs at the 6-turn clinical scan; the start image does not need it).
 __global__ __launch_bounds__(kFbpBlock) void fbp_filter_kernel(
     const float *raw, int64_t nch, int64_t nr, const double *pre_ch,
-    const double *pre_row, const double *godd, double g0, double alpha, float *q)
+    const double *pre_row, const float *godd, double g0, double alpha, float *q)
 {
-    extern __shared__ double line[];
+    extern __shared__ float line[];
     const int64_t view = blockIdx.x / nr, row = blockIdx.x - view * nr;
     const int64_t base = row + nr * nch * view;
     for (int64_t j = threadIdx.x; j < nch; j += kFbpBlock)
-        line[j] = (double)raw[base + nr * j] * pre_ch[j] * pre_row[row];
+        line[j] = (float)((double)raw[base + nr * j] * pre_ch[j] * pre_row[row]);
     __syncthreads();
     for (int64_t j = threadIdx.x; j < nch; j += kFbpBlock) {
-        double acc = g0 * line[j];
+        float acc = (float)g0 * line[j];
         // odd distances only: k = j - 1, j - 3, ... and j + 1, j + 3, ...
         for (int64_t k = j - 1; k >= 0; k -= 2)
             acc += line[k] * godd[j - k];
         for (int64_t k = j + 1; k < nch; k += 2)
             acc += line[k] * godd[k - j];
-        q[base + nr * j] = (float)(alpha * acc);
+        q[base + nr * j] = (float)(alpha * (double)acc);
     }
 }
 
@@ -189,7 +190,9 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
     double *d_tab = nullptr;
     int32_t *d_cols = nullptr;
     int64_t *d_off = nullptr;
+    float *d_godd = nullptr;
     auto cleanup = [&]() {
+        if (d_godd) (void)hipFree(d_godd);
         if (d_raw) (void)hipFree(d_raw);
         if (d_q) (void)hipFree(d_q);
         if (d_x) (void)hipFree(d_x);
@@ -203,13 +206,15 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
         (rc = dev_alloc_n(&d_x, nvox, "volume")) ||
         (rc = dev_alloc_n(&d_tab, 2 * nch + nr, "filter tables")) ||
         (rc = dev_alloc_n(&d_cols, std::max<size_t>(1, cols.size()), "FOV columns")) ||
-        (rc = dev_alloc_n(&d_off, g->nz, "slice offsets"))) {
+        (rc = dev_alloc_n(&d_off, g->nz, "slice offsets")) ||
+        (rc = dev_alloc_n(&d_godd, nch, "ramp kernel"))) {
         cleanup();
         return rc;
     }
     hipError_t e = hipMemcpy(d_raw, raw, ds * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_tab, pre_ch.data(), nch * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_tab + nch, godd.data(), nch * 8, hipMemcpyHostToDevice);
+    std::vector<float> goddf(godd.begin(), godd.end());
+    if (e == hipSuccess) e = hipMemcpy(d_godd, goddf.data(), nch * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_tab + 2 * nch, pre_row.data(), nr * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess && !cols.empty())
         e = hipMemcpy(d_cols, cols.data(), cols.size() * 4, hipMemcpyHostToDevice);
@@ -217,8 +222,8 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
     if (e == hipSuccess) e = hipMemset(d_x, 0, nvox * 4);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(fbp_filter_kernel, dim3((unsigned)(g->readings * nr)),
-                           dim3(kFbpBlock), nch * sizeof(double), 0, d_raw, nch, nr, d_tab,
-                           d_tab + 2 * nch, d_tab + nch, g0, alpha, d_q);
+                           dim3(kFbpBlock), nch * sizeof(float), 0, d_raw, nch, nr, d_tab,
+                           d_tab + 2 * nch, d_godd, g0, alpha, d_q);
         e = hipGetLastError();
     }
     if (e == hipSuccess && !cols.empty()) {
