@@ -35,6 +35,13 @@ def run_configure(path):
     ("config4", 3000, 36, 256, 512, 64, 13824),
     ("tiny", 100, 12, 8, 24, 8, 288),
     ("tiny_ffs", 100, 12, 8, 24, 8, 288),
+    # z flying focal spot: 96 views/turn, beam 16 x 0.22 cm = 3.52 cm, at the
+    # ROI edge 3.52 * (59.5 + 2.4) / 108.56 = 2.007 -> N_sm = int(100 * 2.007)
+    # = 200; 8 projection rows; slices 0.12 / (0.96 / 96) = 12 views
+    ("tiny_zffs", 200, 12, 5, 24, 8, 288),
+    # both spots: 192 views/turn -> int(200 * 2.007) = 401, raised to 408
+    # (even, N_sm/2 divisible by n_ffs = 4); slices 0.12 / 0.005 = 24 views
+    ("tiny_zphiffs", 408, 24, 5, 24, 8, 576),
 ])
 def test_derived_sizes(make_scan, name, nsm, vps, nz, nxy, rows, readings):
     s = make_scan(name)
