@@ -24,6 +24,7 @@
 #include <hip/hip_fp16.h>
 
 #include <cmath>
+#include <string>
 #include <vector>
 
 using namespace ctbb;
@@ -156,6 +157,9 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
         return fail(CTBB_E_STATE, "matrix without view offsets");
     const int64_t nch = g->n_channels, nr = g->n_rows, nsm = g->n_views_sm;
     const int64_t nxy = g->nx * g->ny, nvox = nxy * g->nz, ds = data_size_of(g);
+    if (nch * (int64_t)sizeof(float) > 65536)
+        return fail(CTBB_E_ARG, "detector line of " + std::to_string(nch) +
+                                    " channels exceeds the 64 KiB filter tile");
     const bool ffs = g->zffs || g->phiffs;
     // filter tables (host libm, FP64)
     const double alpha = ffs ? g->fan_angle_increment
@@ -204,7 +208,7 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
     if ((rc = dev_alloc_n(&d_raw, ds, "raw data")) ||
         (rc = dev_alloc_n(&d_q, ds, "filtered sinogram")) ||
         (rc = dev_alloc_n(&d_x, nvox, "volume")) ||
-        (rc = dev_alloc_n(&d_tab, 2 * nch + nr, "filter tables")) ||
+        (rc = dev_alloc_n(&d_tab, nch + nr, "pre-weights")) ||
         (rc = dev_alloc_n(&d_cols, std::max<size_t>(1, cols.size()), "FOV columns")) ||
         (rc = dev_alloc_n(&d_off, g->nz, "slice offsets")) ||
         (rc = dev_alloc_n(&d_godd, nch, "ramp kernel"))) {
@@ -215,7 +219,7 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
     if (e == hipSuccess) e = hipMemcpy(d_tab, pre_ch.data(), nch * 8, hipMemcpyHostToDevice);
     std::vector<float> goddf(godd.begin(), godd.end());
     if (e == hipSuccess) e = hipMemcpy(d_godd, goddf.data(), nch * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_tab + 2 * nch, pre_row.data(), nr * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_tab + nch, pre_row.data(), nr * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess && !cols.empty())
         e = hipMemcpy(d_cols, cols.data(), cols.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_off, off.data(), g->nz * 8, hipMemcpyHostToDevice);
@@ -223,7 +227,7 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
     if (e == hipSuccess) {
         hipLaunchKernelGGL(fbp_filter_kernel, dim3((unsigned)(g->readings * nr)),
                            dim3(kFbpBlock), nch * sizeof(float), 0, d_raw, nch, nr, d_tab,
-                           d_tab + 2 * nch, d_godd, g0, alpha, d_q);
+                           d_tab + nch, d_godd, g0, alpha, d_q);
         e = hipGetLastError();
     }
     if (e == hipSuccess && !cols.empty()) {

@@ -2061,6 +2061,9 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     }
     if (tile < 2)
         return bail(fail(CTBB_E_ARG, "tile must be >= 2"));
+    // every tile of at least max(nx, ny) gives the same single-column
+    // batches in raster order; the clamp bounds the tile x tile loop below
+    tile = std::min<int64_t>(tile, std::max<int64_t>({2, g->nx, g->ny}));
     const double fov_limit = (g->acquisition_fov / 2.0) * (g->acquisition_fov / 2.0);
     auto in_fov = [&](int64_t i, int64_t j) {
         const double y = (j - g->center_voxel_y) * g->voxel_size_y;
