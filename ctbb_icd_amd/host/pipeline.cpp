@@ -113,7 +113,7 @@ void load_raw(struct recon_params *rp, struct ct_data *data)
         // binary frames, unweighted as in the reference
         data->weights = nullptr;
         if (ctbb_read_raw(rp->prm_path.c_str(), data->raw) != CTBB_OK)
-            die(ctbb_last_error());
+            die("Raw data read failed");
         std::cout << "Read " << rp->Readings << " binary frames" << std::endl;
         return;
     }
@@ -198,7 +198,7 @@ void icd_iteration(const struct recon_params *rp, struct ct_data *data)
     std::vector<int64_t> si = slice_index_of(rp, data);
     if (rp->wfbp_initialize && rp->initial_recon_path.compare("") == 0) {
         if (ctbb_fbp(data->ssm, data->raw, si.data(), data->recon_volume) != CTBB_OK)
-            die(ctbb_last_error());
+            die("FBP initialization failed");
         std::cout << "Initialized from the GPU FBP image" << std::endl;
     }
     if (rp->ssm_fp16 && ctbb_ssm_to_fp16(data->ssm) != CTBB_OK)
