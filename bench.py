@@ -176,9 +176,12 @@ def main():
     # each FOV column's SSM entries once per sweep, charged to the gather)
     gather_bytes = visits * 4 + st.ssm_entries * entry_bytes * args.steps / steps
     scatter_bytes = visits * (8 if weighted else 4)
+    # the solve: each voxel update reads and writes its FP32 voxel (the
+    # partial sums it adds up are this design's intermediates, not counted)
+    solve_bytes = st.voxel_updates / steps * 8
     kernels = {
         "icd_gather_kernel": (gather_bytes, st.gather_ms_avg),
-        "icd_solve_kernel": (0.0, st.solve_ms_avg),
+        "icd_solve_kernel": (solve_bytes, st.solve_ms_avg),
         "icd_scatter_kernel": (scatter_bytes, st.scatter_ms_avg),
     }
     dom = max(kernels, key=lambda k: kernels[k][1])
