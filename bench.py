@@ -87,11 +87,18 @@ def time_to_tolerance():
                   r"([0-9.]+) s -> ([0-9.]+)x", f.read_text())
     if not m:
         return None
-    return {"workload": "config4 (512x512x256, 6 turns, count weights), 2 held-out "
-                        "phantoms", "tolerance": 1e-4,
-            "plain_fp32_s": float(m.group(2)), "plain_sweeps": float(m.group(1)),
-            "fp16_learned_warm_s": float(m.group(4)), "warm_sweeps": float(m.group(3)),
-            "speedup": float(m.group(5)), "source": "profiles/warmstart_config4.txt"}
+    out = {"workload": "config4 (512x512x256, 6 turns, count weights), 2 held-out "
+                       "phantoms", "tolerance": 1e-4,
+           "plain_fp32_s": float(m.group(2)), "plain_sweeps": float(m.group(1)),
+           "fp16_learned_warm_s": float(m.group(4)), "warm_sweeps": float(m.group(3)),
+           "speedup": float(m.group(5)), "source": "profiles/warmstart_config4.txt"}
+    # the non-learned comparison: GPU FBP start on the FP16 matrix (one phantom)
+    f2 = re.search(r"# FBP start: ([0-9]+) sweeps / ([0-9.]+) s .* -> ([0-9.]+)x",
+                   f.read_text())
+    if f2:
+        out["fp16_fbp_start"] = {"sweeps": int(f2.group(1)), "seconds": float(f2.group(2)),
+                                 "speedup": float(f2.group(3)), "phantoms": 1}
+    return out
 
 
 def cpu_baseline(scan, ssm, raw, w, target_s: float, threads: int):

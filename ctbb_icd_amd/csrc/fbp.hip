@@ -11,14 +11,17 @@
 //   R(view, ch, row) = y * R_src * cos(gamma_ch) * D / sqrt(D^2 + ap_row^2)
 //   Q(view, ch, row) = alpha * sum_k R(view, k, row) g(ch - k),
 //       g(0) = 1/(8 alpha^2), g(n even) = 0, g(n odd) = -1/(2 pi^2 sin^2(n alpha))
-//   x(c, k) = (2 pi / N_c) * sum over SSM views v of
-//       [sum_e a_e Q(ray_e + off_k)] / [sum_e a_e] / L_v(c)^2
+//   x(c, k) = 2 pi * sum_v w_v P_v / sum_v w_v,
+//   P_v = [sum_e a_e Q(ray_e + off_k)] / [sum_e a_e] / L_v(c)^2
 //
 // with e the entries of column c in view v whose sample lies inside the
-// sinogram (the footprint-weighted value of Q where the voxel projects),
-// N_c the number of views with such entries (redundant coverage -- the SSM
-// window spans more than a turn -- is averaged) and L_v(c) the in-plane
-// distance of the column from the view's nominal source.
+// sinogram (P_v: the footprint-weighted value of Q where the voxel projects)
+// and L_v(c) the in-plane distance of the column from the view's nominal
+// source.  The SSM window spans more than a turn, so a column can see one
+// source angle from two views N_turn apart: w_v = 1 / n_v, n_v the number of
+// views of the column's view range [first, last] at v's source angle, counts
+// every angle once (first / last: the views of the column's first / last
+// entry, clipped to the views whose samples lie in the sinogram).
 #include "common.hpp"
 
 #include <hip/hip_fp16.h>
@@ -72,13 +75,14 @@ __device__ __forceinline__ double fbp_l2(int64_t v, int64_t nsm, double dtheta, 
 // One wavefront per FOV column; lane l takes slices l, l + 64, ...  The
 // column's entries are read once per 64 slices (every lane reads the same
 // entry); each lane gathers Q at its slice's sample.  Sums in FP64.
+// nvr = n_channels * n_rows (samples per view), nturn = views per turn.
 template <bool FP16>
 __global__ __launch_bounds__(kFbpBlock) void fbp_backproject_kernel(
     const int64_t *col_ptr, const int32_t *idx, const float *val, const __half *val16,
     const float *col_scale, const int32_t *view_off, const int32_t *cols, int64_t ncols,
     int64_t nx, int64_t nxy, int64_t nz, int64_t nsm, double cx, double cy, double vx,
     double vy, double R, double dtheta, const int64_t *slice_off,
-    int64_t ds, const float *q, float *x)
+    int64_t ds, int64_t nvr, int64_t nturn, const float *q, float *x)
 {
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = (int64_t)gridDim.x * (kFbpBlock / 64);
@@ -90,15 +94,30 @@ __global__ __launch_bounds__(kFbpBlock) void fbp_backproject_kernel(
         const int64_t b = col_ptr[col], len = col_ptr[col + 1] - b;
         const int32_t *vo = view_off + col * nsm;
         const float sc = FP16 ? col_scale[col] : 1.0f;
+        // views of the column's first and last entry (entries run view by view)
+        const int64_t fv = len > 0 ? (int64_t)idx[b] / nvr : 0;
+        const int64_t lv = len > 0 ? (int64_t)idx[b + len - 1] / nvr : -1;
         for (int64_t k0 = 0; k0 < nz; k0 += 64) {
             const int64_t k = k0 + lane;
             const bool own = k < nz;
             const int64_t off = own ? slice_off[k] : 0;
+            // the slice's views whose samples lie in the sinogram: off is a
+            // whole number of views, so the in-range views are a range
+            const int64_t offv = off / nvr;
+            const int64_t first = fv > -offv ? fv : -offv;
+            const int64_t last = lv < ds / nvr - 1 - offv ? lv : ds / nvr - 1 - offv;
             // the column's entries run view by view; kFbpUnroll of them are
             // loaded and gathered at once, then summed in entry order, a
             // view's footprint sum closing where its entries end
-            double acc = 0.0, sa = 0.0, sq = 0.0;
-            int64_t nv = 0, v = 0;
+            double acc = 0.0, sa = 0.0, sq = 0.0, wsum = 0.0;
+            int64_t v = 0;
+            // closes view v: its weight 1 / n_v (sa > 0 puts v in [first, last])
+            auto close_view = [&]() {
+                const double wv =
+                    1.0 / (double)((last - v) / nturn + (v - first) / nturn + 1);
+                acc += wv * (sq / sa / fbp_l2(v, nsm, dtheta, R, px, py));
+                wsum += wv;
+            };
             int64_t e1 = nsm > 1 ? (int64_t)vo[1] : len;
             for (int64_t e0 = 0; e0 < len; e0 += kFbpUnroll) {
                 double av[kFbpUnroll], qv[kFbpUnroll];
@@ -120,10 +139,8 @@ __global__ __launch_bounds__(kFbpBlock) void fbp_backproject_kernel(
                     if (e >= len)
                         break;
                     while (e >= e1) {     // close view v (and any empty ones)
-                        if (sa > 0.0) {
-                            acc += sq / sa / fbp_l2(v, nsm, dtheta, R, px, py);
-                            nv++;
-                        }
+                        if (sa > 0.0)
+                            close_view();
                         sa = 0.0;
                         sq = 0.0;
                         v++;
@@ -135,12 +152,10 @@ __global__ __launch_bounds__(kFbpBlock) void fbp_backproject_kernel(
                     }
                 }
             }
-            if (sa > 0.0) {
-                acc += sq / sa / fbp_l2(v, nsm, dtheta, R, px, py);
-                nv++;
-            }
+            if (sa > 0.0)
+                close_view();
             if (own)
-                x[col + nxy * k] = nv ? (float)(acc * (2.0 * M_PI / (double)nv)) : 0.0f;
+                x[col + nxy * k] = wsum > 0.0 ? (float)(2.0 * M_PI * acc / wsum) : 0.0f;
         }
     }
 }
@@ -177,6 +192,9 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
         godd[n] = -1.0 / (2.0 * M_PI * M_PI * sn * sn);
     }
     const double g0 = 1.0 / (8.0 * alpha * alpha);
+    const int64_t nturn = (int64_t)std::llround(2.0 * M_PI / g->tube_angle_increment);
+    if (nturn < 1)
+        return fail(CTBB_E_ARG, "tube angle increment above a turn");
     // FOV columns and slice offsets
     const double fov = (g->acquisition_fov / 2.0) * (g->acquisition_fov / 2.0);
     std::vector<int32_t> cols;
@@ -239,14 +257,14 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
                                m->view_off, d_cols, ncols, g->nx, nxy, g->nz, nsm,
                                g->center_voxel_x, g->center_voxel_y, g->voxel_size_x,
                                g->voxel_size_y, Rs, g->tube_angle_increment, d_off,
-                               ds, d_q, d_x);
+                               ds, nch * nr, nturn, d_q, d_x);
         else
             hipLaunchKernelGGL(fbp_backproject_kernel<false>, dim3(grid), dim3(kFbpBlock), 0, 0,
                                m->col_ptr, m->idx, m->val, m->val16, m->col_scale,
                                m->view_off, d_cols, ncols, g->nx, nxy, g->nz, nsm,
                                g->center_voxel_x, g->center_voxel_y, g->voxel_size_x,
                                g->voxel_size_y, Rs, g->tube_angle_increment, d_off,
-                               ds, d_q, d_x);
+                               ds, nch * nr, nturn, d_q, d_x);
         e = hipGetLastError();
     }
     if (e == hipSuccess)

@@ -138,7 +138,8 @@ int ctbb_forward_project(const ctbb_ssm *m, const int64_t *slice_index,
  * by rotate_slices_fixed2rotating: every rotated slice is reconstructed in
  * its own frame by an equiangular fan-beam FBP (ramp filter along channels,
  * cos(gamma) and FDK cone pre-weights) backprojected through the matrix
- * (footprint-weighted per view, 1/L^2 weighted, redundant views averaged).
+ * (footprint-weighted per view, 1/L^2 weighted; views one turn apart share
+ * a source angle and are weighted so that every angle counts once).
  * raw: [readings*n_channels*n_rows] (reference layout); slice_index[nz];
  * volume: host FP32 [nx*ny*nz], rotating frame, zero outside the FOV. */
 int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slice_index,

@@ -12,11 +12,15 @@ cone weight, backprojected through the stored system matrix:
   Q(view, ch, row) = alpha * sum_k R(view, k, row) g(ch - k),
       g(0) = 1 / (8 alpha^2), g(n even) = 0,
       g(n odd) = -1 / (2 pi^2 sin^2(n alpha))
-  x(c, k) = (2 pi / N_c) * sum over SSM views v of
-      (sum_e a_e Q[ray_e + off_k]) / (sum_e a_e) / L_v(c)^2
-  (entries e of column c in view v whose sample lies in the sinogram; N_c:
-  the number of views with such entries, which averages redundant coverage;
-  L_v(c): in-plane distance from the view's nominal source to the column).
+  x(c, k) = 2 pi * sum_v w_v P_v / sum_v w_v,
+      P_v = (sum_e a_e Q[ray_e + off_k]) / (sum_e a_e) / L_v(c)^2
+  over the SSM views v with entries e of column c whose samples lie in the
+  sinogram; L_v(c): in-plane distance from the view's nominal source to the
+  column.  Redundancy weight w_v = 1 / n_v, n_v = the number of views of the
+  column's view range [first, last] at the same source angle as v (views
+  N_turn apart), so every projection angle counts once: first / last = the
+  views of the column's first / last entry, clipped to the views whose
+  samples lie in the sinogram for slice k.
 """
 from __future__ import annotations
 
@@ -41,20 +45,19 @@ def filter_sinogram(g, raw):
     D = g.source_detector_distance
     w = (g.focal_spot_radius * np.cos(gam))[:, None] * (D / np.sqrt(D * D + ap * ap))[None, :]
     R = p * w[None, :, :]
-    n = np.arange(-(nch - 1), nch)
-    h = np.zeros(n.size)
-    h[n == 0] = 1.0 / (8.0 * alpha * alpha)
+    # H[j, k] = g(j - k); Q[v, j, r] = alpha * sum_k H[j, k] R[v, k, r]
+    n = np.arange(nch)[:, None] - np.arange(nch)[None, :]
+    H = np.zeros((nch, nch))
+    H[n == 0] = 1.0 / (8.0 * alpha * alpha)
     odd = (n % 2) != 0
-    h[odd] = -1.0 / (2.0 * np.pi ** 2 * np.sin(n[odd] * alpha) ** 2)
-    Q = np.zeros_like(R)
-    for j in range(nch):
-        # Q[j] = alpha * sum_k R[k] h[j - k]
-        Q[:, j, :] = alpha * np.tensordot(R, h[(j - np.arange(nch)) + (nch - 1)], axes=([1], [0]))
+    H[odd] = -1.0 / (2.0 * np.pi ** 2 * np.sin(n[odd] * alpha) ** 2)
+    Q = alpha * np.matmul(H, R)            # batched over views: (nch, nch) @ (nch, nr)
     return Q
 
 
-def fbp(g, ssm, raw, slice_index):
-    """Rotating-frame FBP volume [nz * ny * nx] (index c + nx*ny*k)."""
+def fbp(g, ssm, raw, slice_index, slices=None):
+    """Rotating-frame FBP volume [nz * ny * nx] (index c + nx*ny*k); with
+    `slices`, only those slices are formed (the others stay zero)."""
     cp, idx, val = ssm
     Q = filter_sinogram(g, raw).reshape(-1)
     ds = Q.size
@@ -70,6 +73,7 @@ def fbp(g, ssm, raw, slice_index):
     x = np.zeros(g.nz * ncol)
     fov = (g.acquisition_fov / 2.0) ** 2
     R = g.focal_spot_radius
+    nturn = int(round(2.0 * np.pi / g.tube_angle_increment))
     for c in range(ncol):
         ix, iy = c % g.nx, c // g.nx
         px = (ix - g.center_voxel_x) * g.voxel_size_x
@@ -84,7 +88,7 @@ def fbp(g, ssm, raw, slice_index):
         th = (view - 0.5 * nsm) * g.tube_angle_increment
         L2 = (R * np.cos(th) - px) ** 2 + (R * np.sin(th) - py) ** 2
         starts = np.flatnonzero(np.r_[True, view[1:] != view[:-1]])
-        for k in range(g.nz):
+        for k in (range(g.nz) if slices is None else slices):
             r = q + offs[k]
             ok = (r >= 0) & (r < ds)
             aq = np.where(ok, a * Q[np.clip(r, 0, ds - 1)], 0.0)
@@ -94,5 +98,10 @@ def fbp(g, ssm, raw, slice_index):
             l2 = L2[starts]
             m = sa > 0
             if m.any():
-                x[c + ncol * k] = np.sum(sq[m] / sa[m] / l2[m]) * (2.0 * np.pi / m.sum())
+                offv = offs[k] // (nch * nr)
+                first = max(int(view[0]), -offv)
+                last = min(int(view[-1]), g.readings - 1 - offv)
+                v = view[starts][m]
+                w = 1.0 / ((last - v) // nturn + (v - first) // nturn + 1)
+                x[c + ncol * k] = 2.0 * np.pi * np.sum(w * sq[m] / sa[m] / l2[m]) / np.sum(w)
     return x
