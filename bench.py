@@ -92,12 +92,12 @@ def time_to_tolerance():
            "plain_fp32_s": float(m.group(2)), "plain_sweeps": float(m.group(1)),
            "fp16_learned_warm_s": float(m.group(4)), "warm_sweeps": float(m.group(3)),
            "speedup": float(m.group(5)), "source": "profiles/warmstart_config4.txt"}
-    # the non-learned comparison: GPU FBP start on the FP16 matrix (one phantom)
+    # the non-learned comparison: GPU FBP start on the FP16 matrix, same phantoms
     f2 = re.search(r"# FBP start: ([0-9]+) sweeps / ([0-9.]+) s .* -> ([0-9.]+)x",
                    f.read_text())
     if f2:
         out["fp16_fbp_start"] = {"sweeps": int(f2.group(1)), "seconds": float(f2.group(2)),
-                                 "speedup": float(f2.group(3)), "phantoms": 1}
+                                 "speedup": float(f2.group(3))}
     return out
 
 

@@ -264,9 +264,10 @@ def _main():
     ap.add_argument("--fp16-warm", action="store_true",
                     help="warm runs on the FP16 SSM, judged with the FP32 model's cost")
     ap.add_argument("--fbp-target", type=float, default=0.0,
-                    help="converged reference cost of seed 2000 from an earlier run: time "
+                    help="reference cost of phantom --fbp-seed from an earlier run: time "
                          "only the FBP start against it (no training, reference or "
                          "learned-start runs)")
+    ap.add_argument("--fbp-seed", type=int, default=2000, help="phantom of --fbp-target")
     ap.add_argument("--workdir", default="/tmp/ctbb_warmstart")
     ap.add_argument("--out", default="", help="JSON summary path")
     ap.add_argument("--params-out", default="", help="trained parameters (.npy)")
@@ -287,7 +288,7 @@ def _main():
     summary = dict(config=a.config, train_config=a.train_config or a.config,
                    fp16_warm=a.fp16_warm, early=a.early, tol=a.tol)
     if a.fbp_target > 0:
-        scan = make_scan(a.config, 2000)
+        scan = make_scan(a.config, a.fbp_seed)
         g = scan.geom
         ssm = SSM.build(g)
         ssm_w = ssm
@@ -302,9 +303,9 @@ def _main():
                 return model_cost(ssm, raw, w, scan.slice_index, opts, rr.volume())
         target = a.fbp_target * (1 + a.tol)
         fr = fbp_run(scan, ssm_w, raw, w, target, a.ref_its, log, cost_fn)
-        log(f"[warmstart] seed 2000: FBP start reaches {a.tol:g} of {a.fbp_target:.10e}: "
+        log(f"[warmstart] seed {a.fbp_seed}: FBP start reaches {a.tol:g} of {a.fbp_target:.10e}: "
             f"{fr.reached} after {fr.iterations} iterations ({fr.seconds:.2f} s)")
-        summary.update(fbp_only=dict(seed=2000, reference_cost=a.fbp_target, target=target,
+        summary.update(fbp_only=dict(seed=a.fbp_seed, reference_cost=a.fbp_target, target=target,
                                      iterations=fr.iterations, seconds=fr.seconds,
                                      reached=fr.reached, costs=fr.costs))
         if a.out:
