@@ -24,6 +24,7 @@ import numpy as np
 from . import Recon, SSM, Scan, UNet, rotate_slices
 
 MU_WATER = 0.193   # 1/cm; inputs and outputs are scaled by 1/MU_WATER
+MU_MAX = 1.0       # 1/cm; network inputs and training images are clipped to [0, MU_MAX]
 
 
 def slice_stacks(vol: np.ndarray) -> np.ndarray:
@@ -81,8 +82,6 @@ def generate_pairs(make_scan, seeds, ssm: SSM, n_early: int, n_target: int,
         log(f"[warmstart] pair seed {s}: {time.perf_counter() - t:.1f} s")
     return pairs
 
-
-MU_MAX = 1.0        # 1/cm; training images are clipped to [0, MU_MAX]
 
 
 def train(net: UNet, pairs: list[Pair], epochs: int, lr: float, seed: int = 0,

@@ -1,6 +1,7 @@
 """CPU checks of the U-Net oracle (oracle/unet_oracle.py): its convolution
 against a direct loop, its maxpool routing, and its gradients against central
-finite differences of its own FP64 loss (no BF16 rounding)."""
+finite differences of its own FP64 loss (no BF16 rounding); and the
+network's input stacking in the warm-start host code."""
 import numpy as np
 
 import unet_oracle as U
@@ -62,3 +63,16 @@ def test_oracle_gradients_match_finite_differences():
         lm = U.forward_backward(pm, inp, tgt, L, base, bf16_round=False)[1]
         fd = (lp - lm) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, g[i])
+
+
+def test_slice_stacks_repeat_the_end_slices():
+    """The network input of slice k is (k-1, k, k+1), the end slices
+    repeated (host logic of the learned warm start)."""
+    from ctbb_icd_amd.warmstart import slice_stacks
+    vol = np.arange(4 * 2 * 3, dtype=np.float32).reshape(4, 2, 3)
+    st = slice_stacks(vol)
+    assert st.shape == (4, 3, 2, 3)
+    for k in range(4):
+        assert np.array_equal(st[k, 1], vol[k])
+        assert np.array_equal(st[k, 0], vol[max(k - 1, 0)])
+        assert np.array_equal(st[k, 2], vol[min(k + 1, 3)])
