@@ -79,14 +79,16 @@ int main(int argc, char **argv)
         if (a != "-t" && a != "-v" && a != "--timing")
             usage();
     }
+    struct ct_data data = {};
+    std::string parameter_file = argv[argc - 1];
+    // the prm is parsed and checked on the host first, so its messages come
+    // out as the reference's do on any machine
+    struct recon_params rp = configure_recon_params(parameter_file);
+    rp.prm_path = parameter_file;
     if (ctbb_init(0) != CTBB_OK) {
         std::cout << "No GPU: " << ctbb_last_error() << std::endl;
         return 1;
     }
-    struct ct_data data = {};
-    std::string parameter_file = argv[argc - 1];
-    struct recon_params rp = configure_recon_params(parameter_file);
-    rp.prm_path = parameter_file;
     load_raw(&rp, &data);
     const struct recon_params rp_const = rp;
     initialize_recon_volume(&rp_const, &data);
