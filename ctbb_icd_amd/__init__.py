@@ -166,6 +166,13 @@ def _p(a: np.ndarray | None):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
+def _need(a: np.ndarray | None, n: int, what: str) -> None:
+    """Size check before a buffer crosses the C ABI: the library reads and
+    writes as many elements as the geometry implies."""
+    if a is not None and a.size != n:
+        raise ValueError(f"{what}: {a.size} elements, expected {n}")
+
+
 def set_verbose(on: bool) -> None:
     lib().ctbb_set_verbose(1 if on else 0)
 
@@ -251,6 +258,9 @@ class SSM:
         col_ptr = np.ascontiguousarray(col_ptr, np.int64)
         idx = np.ascontiguousarray(idx, np.int32)
         val = np.ascontiguousarray(val, np.float32)
+        _need(col_ptr, geom.nx * geom.ny + 1, "col_ptr")
+        _need(idx, int(col_ptr[-1]), "idx")
+        _need(val, int(col_ptr[-1]), "val")
         check(lib().ctbb_ssm_upload(C.byref(geom), _p(col_ptr), _p(idx), _p(val),
                                     C.byref(h)))
         return cls(h, geom)
@@ -298,6 +308,7 @@ class SSM:
                       max_conflict_fraction: float = 0.0) -> int:
         s = C.c_int64()
         si = np.ascontiguousarray(slice_index, np.int64)
+        _need(si, self.geom.nz, "slice_index")
         check(lib().ctbb_ssm_group_spacing(self.h, _p(si), max_spacing,
                                            max_conflict_fraction, C.byref(s)))
         return s.value
@@ -305,6 +316,8 @@ class SSM:
     def forward_project(self, slice_index: np.ndarray, x: np.ndarray) -> np.ndarray:
         si = np.ascontiguousarray(slice_index, np.int64)
         x = np.ascontiguousarray(x, np.float64)
+        _need(si, self.geom.nz, "slice_index")
+        _need(x, self.geom.nvox, "volume")
         y = np.zeros(self.geom.data_size, np.float64)
         check(lib().ctbb_forward_project(self.h, _p(si), _p(x), _p(y)))
         return y
@@ -314,6 +327,8 @@ class SSM:
         si = np.ascontiguousarray(slice_index, np.int64)
         raw = np.ascontiguousarray(raw, np.float32)
         g = self.geom
+        _need(raw, g.data_size, "raw")
+        _need(si, g.nz, "slice_index")
         x = np.zeros(g.nvox, np.float32)
         check(lib().ctbb_fbp(self.h, _p(raw), _p(si), _p(x)))
         return x
@@ -339,7 +354,11 @@ class Recon:
         self._raw = np.ascontiguousarray(raw, np.float32)
         self._w = None if weights is None else np.ascontiguousarray(weights, np.float32)
         self._si = np.ascontiguousarray(slice_index, np.int64)
+        _need(self._raw, self.geom.data_size, "raw")
+        _need(self._w, self.geom.data_size, "weights")
+        _need(self._si, self.geom.nz, "slice_index")
         self.opts = opts
+        self.h = None
         h = C.c_void_p()
         check(lib().ctbb_recon_create(ssm.h, _p(self._raw), _p(self._w), _p(self._si),
                                       C.byref(opts), C.byref(h)))
@@ -347,6 +366,7 @@ class Recon:
 
     def set_volume(self, vol: np.ndarray) -> None:
         vol = np.ascontiguousarray(vol, np.float32)
+        _need(vol, self.geom.nvox, "volume")
         check(lib().ctbb_recon_set_volume(self.h, _p(vol)))
 
     def volume(self) -> np.ndarray:
@@ -396,6 +416,11 @@ def cost(ssm: "SSM", raw, weights, slice_index, opts: IcdOpts, volume) -> float:
     w = None if weights is None else np.ascontiguousarray(weights, np.float32)
     si = np.ascontiguousarray(slice_index, np.int64)
     v = np.ascontiguousarray(volume, np.float32)
+    g = ssm.geom
+    _need(raw, g.data_size, "raw")
+    _need(w, g.data_size, "weights")
+    _need(si, g.nz, "slice_index")
+    _need(v, g.nvox, "volume")
     c = C.c_double()
     check(lib().ctbb_cost(ssm.h, _p(raw), _p(w), _p(si), C.byref(opts), _p(v), C.byref(c)))
     return c.value
@@ -405,6 +430,9 @@ def rotate_slices(geom: Geom, tube_angles, slice_index, volume, direction: int) 
     v = np.ascontiguousarray(volume, np.float32).copy()
     ta = np.ascontiguousarray(tube_angles, np.float64)
     si = np.ascontiguousarray(slice_index, np.int64)
+    _need(v, geom.nvox, "volume")
+    _need(ta, geom.readings, "tube_angles")
+    _need(si, geom.nz, "slice_index")
     check(lib().ctbb_rotate_slices(C.byref(geom), _p(ta), _p(si), _p(v), direction))
     return v
 
@@ -414,6 +442,9 @@ def penalty_steps(geom: Geom, opts: IcdOpts, x, q, alpha, beta, solver: int = 0)
     q = np.ascontiguousarray(q, np.int64)
     a = np.ascontiguousarray(alpha, np.float64)
     b = np.ascontiguousarray(beta, np.float64)
+    _need(x, geom.nvox, "volume")
+    _need(a, q.size, "alpha")
+    _need(b, q.size, "beta")
     du = np.zeros(len(q), np.float64)
     check(lib().ctbb_penalty_steps(C.byref(geom), C.byref(opts), solver, _p(x), len(q), _p(q),
                                    _p(a), _p(b), _p(du)))
@@ -464,6 +495,9 @@ class UNet:
     def infer(self, inp) -> np.ndarray:
         inp = np.ascontiguousarray(inp, np.float32)
         b = inp.shape[0]
+        if inp.shape != (b, 3, self.height, self.width) or not 1 <= b <= self.max_batch:
+            raise ValueError(f"input {inp.shape}: expected (b <= {self.max_batch}, 3, "
+                             f"{self.height}, {self.width})")
         out = np.zeros((b, self.height, self.width), np.float32)
         check(lib().ctbb_unet_infer(self.h, _p(inp), b, _p(out)))
         return out

@@ -53,3 +53,34 @@ def test_empty_slab_is_rejected(ct, tmp_path):
     assert s.geom.nz == 0
     with pytest.raises(ct.CtbbError, match="positive"):
         ct.SSM.build(s.geom)
+
+
+def test_buffer_sizes_are_checked_before_the_abi(ct, make_scan):
+    """The binding refuses buffers whose size disagrees with the geometry
+    before any pointer reaches the library (which trusts the sizes)."""
+    import ctypes
+    import numpy as np
+    s = make_scan("tiny")
+    g = s.geom
+    m = ct.SSM(ctypes.c_void_p(), g)          # never reaches the library
+    x = np.zeros(g.nvox)
+    raw = np.zeros(g.data_size, np.float32)
+    with pytest.raises(ValueError, match="volume"):
+        m.forward_project(s.slice_index, x[:-1])
+    with pytest.raises(ValueError, match="slice_index"):
+        m.forward_project(s.slice_index[:-1], x)
+    with pytest.raises(ValueError, match="raw"):
+        m.fbp(raw[:-1], s.slice_index)
+    with pytest.raises(ValueError, match="raw"):
+        ct.Recon(m, raw[1:], None, s.slice_index, s.opts())
+    with pytest.raises(ValueError, match="weights"):
+        ct.Recon(m, raw, raw[1:], s.slice_index, s.opts())
+    with pytest.raises(ValueError, match="volume"):
+        ct.cost(m, raw, None, s.slice_index, s.opts(), x[1:])
+    with pytest.raises(ValueError, match="tube_angles"):
+        ct.rotate_slices(g, s.tube_angles[1:], s.slice_index, x.astype(np.float32), 1)
+    with pytest.raises(ValueError, match="alpha"):
+        ct.penalty_steps(g, s.opts(), x, np.arange(3), np.zeros(2), np.zeros(3))
+    with pytest.raises(ValueError, match="idx"):
+        ct.SSM.upload(g, np.zeros(g.nx * g.ny + 1, np.int64), np.zeros(1, np.int32),
+                      np.zeros(0, np.float32))
