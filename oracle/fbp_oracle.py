@@ -21,6 +21,11 @@ cone weight, backprojected through the stored system matrix:
   N_turn apart), so every projection angle counts once: first / last = the
   views of the column's first / last entry, clipped to the views whose
   samples lie in the sinogram for slice k.
+
+Pins (tests/test_fbp_oracle.py): water cylinders return mu inside and zero
+outside the FOV; on the config 0 scanner a 10 cm cylinder resolves the ramp
+constant, cos(gamma), 1/L^2 and the redundancy weights.  The FDK cone weight
+is parity unpinned: it moves those checks by < 0.01 %.
 """
 from __future__ import annotations
 
