@@ -13,11 +13,22 @@
  *                        semantics (element-wise order, norm_2 as the plain
  *                        sqrt of the sum of squares) are restated, not run:
  *                        parity with uBLAS itself is unpinned (Boost absent).
+ *                        Flying focal spots (in-plane, z, both): footprint
+ *                        centroids and water-cylinder chords on the FFS scans
+ *                        (a flipped deflection or swapped view parity fails);
+ *                        the deflection constants themselves
+ *                        (src/generate_system_matrix_ffs.cpp:66-67,242-283)
+ *                        are restated, parity unpinned.
  *   orc_forward_project  dense brute-force A x (numpy) on tiny inputs.
  *   orc_quadratic        reference src/penalties.cpp compiled from source
  *   orc_edge_preserving  (oracle/_ref), bit-exact; closed forms.
  *   orc_icd              dense Gauss-Seidel (numpy) with lambda=0; NNLS limit;
  *                        single-voxel du = beta/alpha; cost non-increasing.
+ *   orc_icd_ex           batch surrogate: no batched sweep raises the cost
+ *                        where plain Jacobi curvature does; single-column
+ *                        batches give the reference's curvature bit for bit.
+ *   orc_column_overlaps  is itself the brute force (every entry pair of the
+ *                        two columns); it checks the device's group spacing.
  *   orc_cost             gradient of the cost equals the update optimality
  *                        residual; finite differences.
  *   orc_rotate_slices    reference src/rotate_slices.cpp + src/interp.h

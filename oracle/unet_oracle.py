@@ -4,7 +4,10 @@ TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the GPU
 network's forward pass and gradients.  Same parameter order and layouts as
 the C ABI.  With bf16=True the values the GPU stores in BF16 (input,
 convolution weights, activations, activation gradients) are rounded to BF16
-at the same points; sums are FP64.
+at the same points; sums are FP64.  Pins (tests/test_unet_oracle.py): the
+convolution against a direct loop, the max-pool routing, and the gradients
+of every parameter tensor against central finite differences of its own
+FP64 loss.
 """
 from __future__ import annotations
 
