@@ -27,8 +27,9 @@
  *   orc_icd_ex           batch surrogate: no batched sweep raises the cost
  *                        where plain Jacobi curvature does; single-column
  *                        batches give the reference's curvature bit for bit.
- *   orc_column_overlaps  is itself the brute force (every entry pair of the
- *                        two columns); it checks the device's group spacing.
+ *   orc_column_overlaps  exact by construction (a sorted merge of a column's
+ *                        samples with the same samples dv views later); it
+ *                        checks the device's group spacing.
  *   orc_cost             gradient of the cost equals the update optimality
  *                        residual; finite differences.
  *   orc_rotate_slices    reference src/rotate_slices.cpp + src/interp.h
