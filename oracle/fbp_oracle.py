@@ -60,6 +60,13 @@ def filter_sinogram(g, raw):
     return Q
 
 
+def redundancy_weights(v, first, last, nturn):
+    """1 / n_v for views v of the range [first, last]: n_v = the number of
+    views of the range at v's source angle (views nturn apart)."""
+    v = np.asarray(v, np.int64)
+    return 1.0 / ((last - v) // nturn + (v - first) // nturn + 1)
+
+
 def fbp(g, ssm, raw, slice_index, slices=None):
     """Rotating-frame FBP volume [nz * ny * nx] (index c + nx*ny*k); with
     `slices`, only those slices are formed (the others stay zero)."""
@@ -106,7 +113,6 @@ def fbp(g, ssm, raw, slice_index, slices=None):
                 offv = offs[k] // (nch * nr)
                 first = max(int(view[0]), -offv)
                 last = min(int(view[-1]), g.readings - 1 - offv)
-                v = view[starts][m]
-                w = 1.0 / ((last - v) // nturn + (v - first) // nturn + 1)
+                w = redundancy_weights(view[starts][m], first, last, nturn)
                 x[c + ncol * k] = 2.0 * np.pi * np.sum(w * sq[m] / sa[m] / l2[m]) / np.sum(w)
     return x

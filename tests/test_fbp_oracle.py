@@ -65,3 +65,21 @@ def test_fbp_of_a_large_cylinder_resolves_the_weights(make_scan, orc, fbp_oracle
     assert abs(c - 1.0) < 0.003, c
     assert abs(dev.mean()) < 0.005, dev.mean()
     assert rms < 0.008, rms
+
+
+def test_redundancy_weights_count_every_angle_once(fbp_oracle):
+    """Over any view range, the weights of the views at one source angle
+    sum to 1, so the range's weights add up to its number of distinct
+    angles, min(span, N_turn) (an off-by-one in n_v fails this)."""
+    rng = np.random.default_rng(7)
+    for _ in range(300):
+        nturn = int(rng.integers(1, 40))
+        first = int(rng.integers(0, 50))
+        last = first + int(rng.integers(0, 4 * nturn + 3))
+        v = np.arange(first, last + 1)
+        w = fbp_oracle.redundancy_weights(v, first, last, nturn)
+        for a in range(nturn):
+            sel = (v % nturn) == a
+            if sel.any():
+                assert np.isclose(w[sel].sum(), 1.0), (nturn, first, last, a)
+        assert np.isclose(w.sum(), min(last - first + 1, nturn))
