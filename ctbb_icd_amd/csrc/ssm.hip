@@ -756,6 +756,8 @@ extern "C" int ctbb_ssm_upload(const ctbb_geom *g, const int64_t *col_ptr,
     const int64_t nnz = col_ptr[ncol];
     if (col_ptr[0] != 0 || nnz < 0)
         return fail(CTBB_E_ARG, "col_ptr must start at 0");
+    if (nnz > 0 && (!idx || !val))
+        return fail(CTBB_E_ARG, "null argument");
     const int64_t qmax = g->n_views_sm * g->n_channels * g->n_rows;
     for (int64_t c = 0; c < ncol; c++) {
         if (col_ptr[c + 1] < col_ptr[c])
