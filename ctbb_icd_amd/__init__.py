@@ -437,7 +437,10 @@ def rotate_slices(geom: Geom, tube_angles, slice_index, volume, direction: int) 
     return v
 
 
-def penalty_steps(geom: Geom, opts: IcdOpts, x, q, alpha, beta, solver: int = 0) -> np.ndarray:
+def penalty_steps(geom: Geom, opts: IcdOpts, x, q, alpha, beta,
+                  volume_fp32: bool = False) -> np.ndarray:
+    """Device penalty steps (ctbb_penalty_steps); volume_fp32: evaluate on x
+    rounded to FP32, as the sweep holds the volume."""
     x = np.ascontiguousarray(x, np.float64)
     q = np.ascontiguousarray(q, np.int64)
     a = np.ascontiguousarray(alpha, np.float64)
@@ -446,7 +449,8 @@ def penalty_steps(geom: Geom, opts: IcdOpts, x, q, alpha, beta, solver: int = 0)
     _need(a, q.size, "alpha")
     _need(b, q.size, "beta")
     du = np.zeros(len(q), np.float64)
-    check(lib().ctbb_penalty_steps(C.byref(geom), C.byref(opts), solver, _p(x), len(q), _p(q),
+    check(lib().ctbb_penalty_steps(C.byref(geom), C.byref(opts), int(volume_fp32), _p(x),
+                                   len(q), _p(q),
                                    _p(a), _p(b), _p(du)))
     return du
 

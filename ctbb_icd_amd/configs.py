@@ -20,6 +20,12 @@ the reference's key set.
   config4          512x512x256, 6 rotations, lung texture, I0 = 1e5
   config4_slab     config4's pixels and phantom model on a 32-slice slab
                    (2 rotations): training pairs for the warm start
+  clin_crop        the clinical 64-row detector and cone angle with coarse
+                   sampling (92 channels of 8 clinical channels, 384 views
+                   per turn, 64x64 pixels of 5.6 mm over the clinical field
+                   of view, 24 slices of 0.6 mm): the clinical ray geometry
+                   (obliquity across the field, slices per footprint) at a
+                   size the CPU oracle runs in seconds
 """
 from __future__ import annotations
 
@@ -108,6 +114,12 @@ CONFIGS["config4"] = dict(CONFIGS["config1"], Readings=13824, z_start=-16.5,
                           synthetic_weights=1)
 CONFIGS["config4_slab"] = dict(CONFIGS["config4"], Readings=4608, z_start=-4.0,
                                z_end=-2.05)
+CONFIGS["clin_crop"] = dict(
+    CONFIGS["config1"], Readings=1152, n_channels=92,
+    num_views_per_turn_without_ffs=192, center_channel_non_ffs=45.25,
+    transaxial_detector_spacing=8 * 0.12858, fan_angle_increment=8 * 0.0011844,
+    nx=64, ny=64, z_start=-7.5, z_end=-6.03, synthetic_n_small=6,
+    synthetic_I0=1e5, synthetic_weights=1)
 
 
 def prm_text(name: str, **overrides) -> str:

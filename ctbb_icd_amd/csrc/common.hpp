@@ -111,13 +111,18 @@ struct ctbb_recon {
     float *du = nullptr;        // per-step voxel updates
     double *th2 = nullptr;      // theta2 (batch surrogate curvature) per voxel
     bool th2_valid = false;     //   [nvox], iteration-invariant, first sweep
+    // batch surrogate: S[r] = sum over the voxels of one step (every column
+    // of the batch, every slice of the group) of their matrix values on data
+    // sample r; zero between steps (first sweep only)
+    float *ssum = nullptr;        // [data_size] or null (plain curvature)
+    float *du_m1 = nullptr;       // [(max_batch+8)*nz] all -1: scatter S += a
+    float *du_p1 = nullptr;       // [(max_batch+8)*nz] all +1: scatter S -= a
     // view-blocked batch copy of the matrix (gather): per batch b and view
     // block ib (kViewBlock views) the columns ranked by detector position,
     // in chunks of 64 ranks; a chunk's entries ordered (view in block, rank)
     int32_t *gb_idx = nullptr;    // [fov entries] ray index (as in the SSM)
     float *gb_val = nullptr;      // [fov entries] FP32 value (FP32 matrices)
     __half *gb_val16 = nullptr;   // [fov entries] FP16 value (FP16 matrices)
-    float *ssum = nullptr;        // [n_sm*nch*nr] batch surrogate ray sums
     uint8_t *gb_key = nullptr;    // [fov entries] view in block << 6 | rank % 64
     uint16_t *gb_perm = nullptr;  // [nib*ncols_fov] column slot of (b, ib, rank)
     int64_t *gb_ptr = nullptr;    // [chunks+1] start of (b, ib, 64-rank chunk)
@@ -147,5 +152,9 @@ struct ctbb_recon {
     double *partials = nullptr; // reduction partials
     int iter_done = 0;
     hipStream_t stream = nullptr;
+    // timing events of ctbb_recon_iterate (created once with the context):
+    // sweep start/end and, per sampled step, around gather / solve / scatter
+    hipEvent_t ev_sweep[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> ev_step;
     std::vector<int64_t> h_slice_index;
 };

@@ -152,8 +152,11 @@ __device__ __forceinline__ double penalty_step(int penalty, int64_t q,
     return edge_preserving_step(q, alpha, beta, lambda, delta, x, nx, n);
 }
 
+// One thread per voxel; X = double (the FP64 volume as given) or float (the
+// volume as the sweep kernel holds it: the sweep calls penalty_step<float>).
+template <typename X>
 __global__ void penalty_steps_kernel(int penalty, double lambda, double delta,
-                                     const double *x, int64_t nx, int64_t nvox,
+                                     const X *x, int64_t nx, int64_t nvox,
                                      int64_t n, const int64_t *q,
                                      const double *alpha, const double *beta,
                                      double *du)
@@ -205,77 +208,6 @@ __device__ __forceinline__ float ssm_value(const SweepArgs &a, int64_t p,
     return FP16 ? __half2float(a.val16[p]) * scale : a.val[p];
 }
 
-// Edge-preserving step by 65-section on one wavefront: the bracket of
-// src/penalties.cpp:106-118 is cut at 64 interior points per round, one per
-// lane; f is non-decreasing in c (alpha >= 0, psi increasing), so the first
-// lane with f > 0 bounds the root from above.  Stops when the half-bracket
-// is below 1e-8 (the reference's tolerance) or on an exact zero, and returns
-// the bracket midpoint: within 1e-8 of the root, like the reference's
-// bisection, in 5 rounds instead of ~27.  Called by all 64 lanes.
-__device__ double edge_preserving_wave(int64_t q, double alpha, double beta,
-                                       double lambda, double delta,
-                                       const float *x, int64_t nx, int64_t n,
-                                       int lane)
-{
-    if (alpha == 0.0)
-        return 0.0;
-    const int64_t nb[8] = {q - nx - 1, q - nx,     q - nx + 1, q - 1,
-                           q + 1,      q + nx - 1, q + nx,     q + nx + 1};
-    const double xq = (double)x[q];
-    double diff[8];
-    bool used[8];
-    double a = beta / alpha, b = beta / alpha;
-    for (int i = 0; i < 8; i++) {
-        used[i] = nb[i] >= 0 && nb[i] < n;
-        diff[i] = used[i] ? ((double)x[nb[i]] - xq) : 0.0;
-        if (used[i]) {
-            if (diff[i] < a) a = diff[i];
-            if (diff[i] > b) b = diff[i];
-        }
-    }
-    const double tol = 0.00000001;
-    for (int round = 0; round < 64 && !((b - a) / 2 < tol); round++) {
-        const double c = a + (b - a) * (double)(lane + 1) / 65.0;
-        const double f = ep_func(c, alpha, beta, delta, lambda, diff, used);
-        const uint64_t zero = __ballot(f == 0.0);
-        if (zero)
-            return __shfl(c, __ffsll((unsigned long long)zero) - 1, 64);
-        const uint64_t pos = __ballot(f > 0.0);
-        if (pos == 0) {
-            a = __shfl(c, 63, 64);
-        } else {
-            const int l = __ffsll((unsigned long long)pos) - 1;
-            const double cl = __shfl(c, l, 64);
-            const double cp = __shfl(c, l > 0 ? l - 1 : 0, 64);
-            if (l > 0)
-                a = cp;
-            b = cl;
-        }
-    }
-    return (a + b) / 2;
-}
-
-// One wavefront per voxel: the sweep's solver, for checking.
-__global__ void penalty_steps_wave_kernel(int penalty, double lambda,
-                                          double delta, const float *x,
-                                          int64_t nx, int64_t nvox, int64_t n,
-                                          const int64_t *q, const double *alpha,
-                                          const double *beta, double *du)
-{
-    const int64_t i = (int64_t)blockIdx.x;
-    const int lane = threadIdx.x & 63;
-    if (i >= n)
-        return;
-    double d;
-    if (penalty == CTBB_PENALTY_QUADRATIC)
-        d = quadratic_step(q[i], alpha[i], beta[i], lambda, x, nx, nvox);
-    else
-        d = edge_preserving_wave(q[i], alpha[i], beta[i], lambda, delta, x, nx,
-                                 nvox, lane);
-    if (lane == 0)
-        du[i] = d;
-}
-
 // Inclusive sum over runs of equal keys across the lanes of a wavefront:
 // bit l of `head` is set where lane l starts a run (lane 0 always does), and
 // runs are contiguous because keys arrive sorted.  After the call, the last
@@ -309,6 +241,8 @@ __device__ __forceinline__ bool run_end(uint64_t head, int lane)
 }
 
 constexpr int kXcd = 8;   // workgroups are dealt round-robin to the 8 XCDs
+// Steps per iteration whose three launches are timed with HIP events.
+constexpr int kTimedSteps = 128;
 
 struct StepArgs {
     int64_t c0, nc, j, ng;
@@ -371,8 +305,8 @@ struct GatherArgs {
     float *part1, *part2;     // [ng][nib][nc]
     int64_t nib;
     const int64_t *si_half;   // [nz] central view - n_sm/2
-    const float *ssum;        // batch surrogate: sum of the batch's values
-                              //   per SSM ray (null: sum w a^2)
+    const float *ssum;        // batch surrogate: sum of the step's values per
+                              //   data sample (null: sum w a^2)
     int64_t nsc;              // workgroups per view block
     int64_t nsm, readings;
 };
@@ -449,7 +383,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                 accm[m][wave * kAcc + q * 64 + lane] = 0.0f;
         for (int64_t p0 = lo; p0 < hi; p0 += 64 * kGatherUnroll) {
             int32_t iv[kGatherUnroll], kv[kGatherUnroll];
-            float av[kGatherUnroll], sv[kGatherUnroll];
+            float av[kGatherUnroll];
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; u++) {
                 const int64_t p = p0 + u * 64 + lane;
@@ -458,12 +392,8 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                 av[u] = ok ? gb_value<FP16>(g, p) : 0.0f;
                 kv[u] = ok ? (int32_t)g.key[p] : -1;
             }
-            if (THETA2) {
-#pragma unroll
-                for (int u = 0; u < kGatherUnroll; u++)
-                    sv[u] = g.ssum ? g.ssum[iv[u]] : av[u];
-            }
             float ev[M][kGatherUnroll], mv[M][kGatherUnroll], wv[M][kGatherUnroll];
+            float sv[M][kGatherUnroll];
 #pragma unroll
             for (int m = 0; m < M; m++)
 #pragma unroll
@@ -474,6 +404,8 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                     mv[m][u] = in ? av[u] : 0.0f;
                     ev[m][u] = ew[in ? r : 0];
                     wv[m][u] = (THETA2 && wt) ? wt[in ? r : 0] : 1.0f;
+                    // surrogate: the step's summed values on this sample
+                    sv[m][u] = THETA2 ? (g.ssum ? g.ssum[in ? r : 0] : av[u]) : 0.0f;
                 }
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; u++) {
@@ -485,7 +417,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                 for (int m = 0; m < M; m++) {
                     t1[m] = run_sum(mv[m][u] * ev[m][u], head, lane);
                     if (THETA2)
-                        t2[m] = run_sum((wv[m][u] * mv[m][u]) * sv[u], head, lane);
+                        t2[m] = run_sum((wv[m][u] * mv[m][u]) * sv[m][u], head, lane);
                 }
                 if (kv[u] >= 0 && run_end(head, lane)) {
 #pragma unroll
@@ -578,8 +510,8 @@ __global__ __launch_bounds__(kSweepBlock) void icd_solve_kernel(SweepArgs a,
     if (FP16) {
         const double sc = (double)a.col_scale[col];
         th1 *= sc;
-        // sum w a^2 carries the scale twice; the surrogate's ray sums are
-        // formed from scaled values, so sum w a s carries it once
+        // sum w a^2 carries the scale twice; the surrogate's sample sums
+        // are formed from scaled values, so sum w a S carries it once
         th2 *= g.ssum ? sc : sc * sc;
     }
     if (THETA2)
@@ -1517,7 +1449,7 @@ extern "C" int ctbb_forward_project(const ctbb_ssm *m,
 }
 
 extern "C" int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *o,
-                                  int solver, const double *x, int64_t n,
+                                  int volume_fp32, const double *x, int64_t n,
                                   const int64_t *q, const double *alpha,
                                   const double *beta, double *du)
 {
@@ -1550,8 +1482,8 @@ extern "C" int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *o,
     if (e == hipSuccess) e = hipMemcpy(d_a, alpha, n * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_b, beta, n * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_q, q, n * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess && solver == 0) {
-        hipLaunchKernelGGL(penalty_steps_kernel, dim3((n + 255) / 256), dim3(256),
+    if (e == hipSuccess && volume_fp32 == 0) {
+        hipLaunchKernelGGL(penalty_steps_kernel<double>, dim3((n + 255) / 256), dim3(256),
                            0, 0, o->penalty, o->lambda, o->delta, d_x, g->nx,
                            nvox, n, d_q, d_a, d_b, d_du);
         e = hipGetLastError();
@@ -1563,8 +1495,8 @@ extern "C" int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *o,
         if ((rc = dev_alloc_n(&d_xf, nvox, "volume (FP32)")) == CTBB_OK) {
             e = hipMemcpy(d_xf, xf.data(), nvox * 4, hipMemcpyHostToDevice);
             if (e == hipSuccess) {
-                hipLaunchKernelGGL(penalty_steps_wave_kernel, dim3((unsigned)n),
-                                   dim3(64), 0, 0, o->penalty, o->lambda, o->delta,
+                hipLaunchKernelGGL(penalty_steps_kernel<float>, dim3((n + 255) / 256),
+                                   dim3(256), 0, 0, o->penalty, o->lambda, o->delta,
                                    d_xf, g->nx, nvox, n, d_q, d_a, d_b, d_du);
                 e = hipGetLastError();
             }
@@ -1604,6 +1536,12 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
     if (r->th2) (void)hipFree(r->th2);
     if (r->part1) (void)hipFree(r->part1);
     if (r->ssum) (void)hipFree(r->ssum);
+    if (r->du_m1) (void)hipFree(r->du_m1);
+    if (r->du_p1) (void)hipFree(r->du_p1);
+    for (hipEvent_t ev : r->ev_sweep)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : r->ev_step)
+        if (ev) (void)hipEventDestroy(ev);
     if (r->part2) (void)hipFree(r->part2);
     if (r->gb_idx) (void)hipFree(r->gb_idx);
     if (r->gb_val) (void)hipFree(r->gb_val);
@@ -1614,40 +1552,6 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return CTBB_OK;
-}
-
-// Batch surrogate: s[ray] = sum of batch b's matrix values on each SSM ray
-// it touches, from the transposed copy (a (view, channel) segment is sorted
-// by row: runs of one row are contiguous, summed in entry order).  Rays the
-// batch does not touch are not written and never read.
-__global__ void batch_ray_sum_kernel(const uint32_t *bent, const float *bval,
-                                     const int32_t *vc_idx, const int64_t *ent_off,
-                                     int64_t b, int64_t nvc, int64_t nch, int64_t nr,
-                                     float *s)
-{
-    const uint32_t *ent = bent + ent_off[b];
-    const float *val = bval + ent_off[b];
-    const int32_t *vci = vc_idx + b * nvc;
-    const int64_t nseg = nvc - 1;
-    for (int64_t seg = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; seg < nseg;
-         seg += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t lo = vci[seg], hi = vci[seg + 1];
-        const int64_t rbase = (seg % nch) % kScatterCh * nr;   // lkey of row 0
-        int64_t cur = -1;
-        float acc = 0.0f;
-        for (int64_t p = lo; p < hi; p++) {
-            const int64_t row = (int64_t)bt_lkey(ent[p]) - rbase;
-            if (row != cur) {
-                if (cur >= 0)
-                    s[seg * nr + cur] = acc;
-                cur = row;
-                acc = 0.0f;
-            }
-            acc += val[p];
-        }
-        if (cur >= 0)
-            s[seg * nr + cur] = acc;
-    }
 }
 
 // View-blocked batch copy (gather).  Per batch b and view block ib the
@@ -2043,6 +1947,13 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     hipError_t he = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
     if (he != hipSuccess)
         return bail(fail(CTBB_E_HIP, std::string("stream: ") + hipGetErrorString(he)));
+    r->ev_step.assign(4 * kTimedSteps, nullptr);
+    for (hipEvent_t *ev : {&r->ev_sweep[0], &r->ev_sweep[1]})
+        if ((he = hipEventCreate(ev)) != hipSuccess)
+            return bail(fail(CTBB_E_HIP, std::string("events: ") + hipGetErrorString(he)));
+    for (auto &ev : r->ev_step)
+        if ((he = hipEventCreate(&ev)) != hipSuccess)
+            return bail(fail(CTBB_E_HIP, std::string("events: ") + hipGetErrorString(he)));
 
     // Columns inside the acquisition FOV (src/icd_iteration.cpp:154-181),
     // grouped into batches: the image is cut into tile x tile blocks of
@@ -2184,13 +2095,27 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
             (rc = dev_alloc_n(&r->th2, r->nvox, "theta2")))
             return bail(rc);
         r->th2_valid = false;
-        // batch surrogate ray sums (one batch at a time, first sweep only);
-        // single-column batches: the surrogate is sum w a^2 itself
+        // batch surrogate sample sums (one step at a time, first sweep only):
+        // zeroed once here, filled and emptied by the +-1 scatters of a step;
+        // single-column batches keep the plain curvature (their conflict-free
+        // slice groups share no sample, where S would equal a)
         const bool single = tile >= std::max(g->nx, g->ny);
-        if (!opts->plain_curvature && !single &&
-            (rc = dev_alloc_n(&r->ssum, g->n_views_sm * g->n_channels * g->n_rows,
-                              "surrogate ray sums")))
-            return bail(rc);
+        if (!opts->plain_curvature && !single) {
+            const int64_t nd = (max_batch + 8) * g->nz;
+            if ((rc = dev_alloc_n(&r->ssum, r->data_size, "surrogate sample sums")) ||
+                (rc = dev_alloc_n(&r->du_m1, nd, "surrogate scatter weights")) ||
+                (rc = dev_alloc_n(&r->du_p1, nd, "surrogate scatter weights")))
+                return bail(rc);
+            std::vector<float> m1(nd, -1.0f), p1(nd, 1.0f);
+            hipError_t he2 = hipMemset(r->ssum, 0, r->data_size * 4);
+            if (he2 == hipSuccess)
+                he2 = hipMemcpy(r->du_m1, m1.data(), nd * 4, hipMemcpyHostToDevice);
+            if (he2 == hipSuccess)
+                he2 = hipMemcpy(r->du_p1, p1.data(), nd * 4, hipMemcpyHostToDevice);
+            if (he2 != hipSuccess)
+                return bail(fail(CTBB_E_HIP, std::string("surrogate set-up: ") +
+                                                 hipGetErrorString(he2)));
+        }
         // scatter dispatch stride: median view distance of consecutive members
         r->view_stride.assign(std::max<int64_t>(1, r->spacing), 1);
         for (int64_t j = 0; j < r->spacing && j < g->nz; j++) {
@@ -2409,19 +2334,15 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
     const int64_t ncg = (r->g.n_channels + kScatterCh - 1) / kScatterCh;
     const size_t scatter_lds = sizeof(float) * kScatterViews * kScatterCh * r->g.n_rows;
 
-    hipEvent_t e0, e1;
-    CTBB_HIP(hipEventCreate(&e0));
-    CTBB_HIP(hipEventCreate(&e1));
+    hipEvent_t e0 = r->ev_sweep[0], e1 = r->ev_sweep[1];
     double sweep_ms = 0.0, resync_ms = 0.0;
     int64_t launches = 0;
     // Per-kernel durations: events around the three launches of every
-    // `every`-th step (at most kSamples steps per iteration).
-    constexpr int kSamples = 128;
+    // `every`-th step (at most kTimedSteps steps per iteration).
+    constexpr int kSamples = kTimedSteps;
     const int64_t steps_per_it = r->nbatch * std::min<int64_t>(r->spacing, r->g.nz);
     const int64_t every = std::max<int64_t>(1, steps_per_it / kSamples);
-    std::vector<hipEvent_t> ev(4 * kSamples);
-    for (auto &x : ev)
-        CTBB_HIP(hipEventCreate(&x));
+    const std::vector<hipEvent_t> &ev = r->ev_step;
     double g_ms = 0.0, v_ms = 0.0, s_ms = 0.0;
     int64_t n_samp = 0;
     for (int it = 0; it < n; it++) {
@@ -2439,13 +2360,6 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
         int64_t step = 0, used = 0;
         for (int64_t b = 0; b < r->nbatch; b++) {
             const int64_t c0 = r->h_batch_ptr[b], nc = r->h_batch_ptr[b + 1] - c0;
-            if (first && r->ssum) {
-                const int64_t nvc = r->g.n_views_sm * r->g.n_channels + 1;
-                hipLaunchKernelGGL(batch_ray_sum_kernel,
-                                   dim3((unsigned)std::min<int64_t>(8192, (nvc + 255) / 256)),
-                                   dim3(256), 0, r->stream, r->bt_ent, r->bt_val, r->bt_vc,
-                                   r->bt_off, b, nvc, r->g.n_channels, r->g.n_rows, r->ssum);
-            }
             for (int64_t j = 0; j < r->spacing && j < r->g.nz; j++) {
                 StepArgs sa;
                 sa.c0 = c0;
@@ -2465,6 +2379,32 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                 }
                 vlo = std::max<int64_t>(vlo, 0);
                 vhi = std::min<int64_t>(vhi, r->g.readings);
+                ScatterOrder so{};
+                int64_t scatter_grid = 0;
+                if (vhi > vlo) {
+                    so.vlo = vlo;
+                    so.vhi = vhi;
+                    so.nq = (vhi - vlo + kScatterViews - 1) / kScatterViews;
+                    so.ncg = ncg;
+                    so.P = std::min<int64_t>(
+                        std::max<int64_t>(1, (r->view_stride[j] + kScatterViews / 2) /
+                                                 kScatterViews),
+                        so.nq);
+                    so.rows = (so.nq + so.P - 1) / so.P;
+                    const int64_t nwg = so.P * so.rows * ncg;
+                    scatter_grid = kXcd * ((nwg + kXcd - 1) / kXcd);
+                }
+                // first sweep with the batch surrogate: S += a over every voxel
+                // of the step (a scatter of -1 per voxel into the zeroed S)
+                SweepArgs as = a;
+                as.e = r->ssum;
+                as.w = nullptr;
+                if (first && r->ssum && scatter_grid > 0) {
+                    as.du = r->du_m1;
+                    hipLaunchKernelGGL(icd_scatter_kernel, dim3((unsigned)scatter_grid),
+                                       dim3(kSweepBlock), scatter_lds, r->stream, as, sa, bt,
+                                       b, so);
+                }
                 const bool samp = (step % every) == 0 && used < kSamples;
                 if (samp)
                     CTBB_HIP(hipEventRecord(ev[4 * used], r->stream));
@@ -2477,26 +2417,20 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                                    dim3(kSweepBlock), 0, r->stream, a, sa, ga);
                 if (samp)
                     CTBB_HIP(hipEventRecord(ev[4 * used + 2], r->stream));
-                if (vhi > vlo) {
-                    ScatterOrder so;
-                    so.vlo = vlo;
-                    so.vhi = vhi;
-                    so.nq = (vhi - vlo + kScatterViews - 1) / kScatterViews;
-                    so.ncg = ncg;
-                    so.P = std::min<int64_t>(
-                        std::max<int64_t>(1, (r->view_stride[j] + kScatterViews / 2) /
-                                                 kScatterViews),
-                        so.nq);
-                    so.rows = (so.nq + so.P - 1) / so.P;
-                    const int64_t nwg = so.P * so.rows * ncg;
-                    hipLaunchKernelGGL(icd_scatter_kernel,
-                                       dim3((unsigned)(kXcd * ((nwg + kXcd - 1) / kXcd))),
+                if (scatter_grid > 0)
+                    hipLaunchKernelGGL(icd_scatter_kernel, dim3((unsigned)scatter_grid),
                                        dim3(kSweepBlock), scatter_lds, r->stream, a, sa, bt, b,
                                        so);
-                }
                 if (samp) {
                     CTBB_HIP(hipEventRecord(ev[4 * used + 3], r->stream));
                     used++;
+                }
+                // ... and S -= a in the same order: exactly zero again
+                if (first && r->ssum && scatter_grid > 0) {
+                    as.du = r->du_p1;
+                    hipLaunchKernelGGL(icd_scatter_kernel, dim3((unsigned)scatter_grid),
+                                       dim3(kSweepBlock), scatter_lds, r->stream, as, sa, bt,
+                                       b, so);
                 }
                 launches += 3;
                 step++;
@@ -2529,10 +2463,6 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
             resync_ms += ms;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    for (auto &x : ev)
-        (void)hipEventDestroy(x);
     if (stats) {
         stats->gather_ms_avg = n_samp ? g_ms / n_samp : 0.0;
         stats->solve_ms_avg = n_samp ? v_ms / n_samp : 0.0;

@@ -56,7 +56,7 @@ static void empty_parameter_file()
            "synthetic:          # 1: generate seeded raw data instead of reading\n"
            "ssm_fp16:           # 1: FP16 matrix values with FP32 column scales\n"
            "resync_every:       # FP64 error-sinogram resync period (iterations)\n"
-           "tile:               # batch tile of the parallel sweep (default 8)\n"
+           "tile:               # batch tile of the parallel sweep (default 4)\n"
            "plain_curvature:    # 1: plain Jacobi curvature in batches (no surrogate)\n"
         << std::endl;
     exit(0);

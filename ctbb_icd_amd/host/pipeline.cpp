@@ -237,6 +237,19 @@ void icd_iteration(const struct recon_params *rp, struct ct_data *data)
         f.write((char *)xd.data(), nv * sizeof(double));
     };
     write_iterate(0);
+    if (dump) {
+        // the sinogram estimate A x0 in FP64 (src/icd_iteration.cpp:127-133).
+        // It is always the projection of the start volume here; the
+        // reference projects only for a wFBP start (:63) and otherwise
+        // writes zeros, also for a start volume read from a file.
+        const size_t ds = rp->Readings * rp->n_channels * rp->Nrows_projection;
+        std::vector<double> yhat(ds);
+        if (ctbb_forward_project(data->ssm, si.data(), xd.data(), yhat.data()) != CTBB_OK)
+            die("sinogram estimate failed");
+        std::ofstream f(dump_dir + "/sino_estimation.rcn", std::ios_base::binary);
+        f.write((char *)yhat.data(), ds * sizeof(double));
+        std::cout << "Wrote initial sinogram to disk." << std::endl;
+    }
     for (size_t n = 0; n < rp->num_iterations; n++) {
         std::cout << "Iteration #" << n + 1 << std::endl;
         ctbb_icd_stats st;

@@ -157,7 +157,9 @@ typedef struct ctbb_recon ctbb_recon;
 
 typedef struct ctbb_icd_opts {
     int32_t penalty;          /* CTBB_PENALTY_*                          */
-    int32_t neighbours;       /* 8 (reference in-plane stencil) or 26    */
+    int32_t neighbours;       /* 8: the reference's in-plane stencil
+                                 (src/penalties.cpp:39-59); the only value
+                                 accepted                                */
     double lambda;            /* regulariser weight (prm `lambda`)       */
     double delta;             /* edge-preserving delta (prm `delta`)     */
     int64_t group_spacing;    /* s; 0 = ctbb_ssm_group_spacing with conflict
@@ -181,17 +183,22 @@ typedef struct ctbb_icd_opts {
                                  i.e. the reference's raster order. */
     int32_t plain_curvature;  /* 0 (default): batch surrogate -- the
                                  curvature of a voxel's update is
-                                 sum_i w_i a_i s_i, s_i = sum over the
-                                 batch's columns of their values on ray i,
-                                 which majorises the data term of the whole
-                                 batch separably, so every step lowers the
-                                 cost however strongly the batch's columns
-                                 share rays (no reference counterpart; equal
-                                 to the reference's sum_i w_i a_i^2 for
-                                 single-column batches).  1: sum_i w_i a_i^2
-                                 in every batch (plain Jacobi across the
-                                 batch; can overshoot when the weighted
-                                 data term dominates the penalty). */
+                                 sum_r w_r a_r S_r over its data samples r,
+                                 S_r = sum over every voxel of the step
+                                 (all the batch's columns, all the group's
+                                 slices) of their values on sample r, which
+                                 majorises the data term of the whole step
+                                 separably (Cauchy-Schwarz), so every step
+                                 lowers the cost however the step's voxels
+                                 share samples -- across columns, across
+                                 slices, and inside a column where the
+                                 default spacing lets footprints overlap
+                                 (no reference counterpart).  Single-column
+                                 batches use the reference's sum_r w_r a_r^2
+                                 (their default spacing is conflict-free).
+                                 1: sum_r w_r a_r^2 in every batch (plain
+                                 Jacobi across the step; can overshoot when
+                                 the weighted data term dominates). */
 } ctbb_icd_opts;
 
 typedef struct ctbb_icd_stats {
@@ -235,8 +242,8 @@ int ctbb_recon_resync(ctbb_recon *r);
  * quadratic:  0.5 sum w e^2 + 0.5 lambda sum_pairs w_d (x_q - x_{q+d})^2
  * edge-pres.:     sum w e^2 +     lambda sum_pairs w_d rho(x_q - x_{q+d})
  * rho(t) = |t| - delta ln(1 + |t|/delta); pairs d in {+1, nx-1, nx, nx+1}
- * with q+d < nx*ny*nz (8-neighbour) and the matching forward half of the
- * 26-neighbour stencil. */
+ * with q+d < nx*ny*nz (the forward half of the 8-neighbour flat-index
+ * stencil, weights 1, 1/sqrt(2), 1, 1/sqrt(2)). */
 int ctbb_recon_cost(ctbb_recon *r, double *cost);
 
 /* Cost of the current volume from the kept FP32 error sinogram, without a
@@ -273,12 +280,13 @@ int ctbb_rotate_slices(const ctbb_geom *g, const double *tube_angles,
 /* ---- penalty steps (for checking the fused device penalty) -------------
  * Evaluates the per-voxel penalty step of src/penalties.cpp:61-154 on the
  * device for n voxels q[i] of volume x (FP64 [nx*ny*nz]) with data terms
- * alpha[i], beta[i]; writes du[i] (before the positivity clamp).
- * solver 0: the reference's bisection, one thread per voxel, FP64 x;
- * solver 1: the sweep kernel's solver (one wavefront per voxel, 65-section
- * for the edge-preserving step) on x rounded to FP32, as the sweep sees it. */
+ * alpha[i], beta[i]; writes du[i] (before the positivity clamp).  The step
+ * is the device function the sweep's solve kernel calls (quadratic closed
+ * form; edge-preserving: the reference's bisection in FP64, tolerance 1e-8),
+ * one thread per voxel.  volume_fp32 = 0: on x as given (FP64);
+ * volume_fp32 = 1: on x rounded to FP32, the volume the sweep holds. */
 int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *opts,
-                       int solver, const double *x, int64_t n,
+                       int volume_fp32, const double *x, int64_t n,
                        const int64_t *q, const double *alpha,
                        const double *beta, double *du);
 

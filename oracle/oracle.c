@@ -147,14 +147,80 @@ static void orc_dir(const ctbb_geom *g, const view_geom *v, int ffs, int j,
 
 typedef void (*emit_fn)(void *ctx, int64_t col, int32_t q, float val);
 
+/* One crossing of ray d (origin v->s) with the voxel line of index m along
+ * its major axis (ix = m for x-major rays, iy = m for y-major rays):
+ * src/generate_system_matrix.cpp:116-184 for one loop step.  Returns 0 when
+ * the reference pushes nothing; else 1 with the crossed minor index *lo (the
+ * two entries go to lo and lo + 1) and their FP32 values.  Shared by the
+ * ray-driven build and the column-subset build, so both evaluate the same
+ * expressions in the same order. */
+static int orc_crossing(const ctbb_geom *g, const view_geom *v,
+                        const double d[3], int xmajor, int m, int *lo,
+                        float *v_lo, float *v_hi)
+{
+    int nx = (int)g->nx, ny = (int)g->ny;
+    double aax = fabs(d[0]), aay = fabs(d[1]);
+    if (xmajor) {
+        double x = (m - g->center_voxel_x) * g->voxel_size_x;
+        double y_hat = v->s[1] + d[1] / d[0] * (x - v->s[0]);
+        double j_hat = y_hat / g->voxel_size_y + g->center_voxel_y;
+        int jy = (int)floor(j_hat);
+        double z_hat = v->s[2] + d[2] / d[0] * (x - v->s[0]);
+        double k_hat = z_hat / g->voxel_size_z + g->center_voxel_z;
+        int kz = (int)floor(k_hat);
+        if (!(jy >= 0 && jy <= ny - 2))
+            return 0;
+        double scale = g->voxel_size_x / aax;
+        double tri;
+        if (kz == -1)
+            tri = scale * (k_hat - kz);
+        else if (kz == 0)
+            tri = scale * (1 - (k_hat - kz));
+        else
+            return 0;
+        *lo = jy;
+        *v_lo = (float)(tri * (1 - (j_hat - jy)));
+        *v_hi = (float)(tri * (j_hat - jy));
+        return 1;
+    } else {
+        double y = (m - g->center_voxel_y) * g->voxel_size_y;
+        double x_hat = v->s[0] + d[0] / d[1] * (y - v->s[1]);
+        double i_hat = x_hat / g->voxel_size_x + g->center_voxel_x;
+        int ix = (int)floor(i_hat);
+        double z_hat = v->s[2] + d[2] / d[1] * (y - v->s[1]);
+        double k_hat = z_hat / g->voxel_size_z + g->center_voxel_z;
+        int kz = (int)floor(k_hat);
+        if (!(ix >= 0 && ix <= nx - 2))
+            return 0;
+        double scale = g->voxel_size_y / aay;
+        double tri;
+        if (kz == -1)
+            tri = scale * (k_hat - kz);
+        else if (kz == 0)
+            tri = scale * (1 - (k_hat - kz));
+        else
+            return 0;
+        *lo = ix;
+        *v_lo = (float)(tri * (1 - (i_hat - ix)));
+        *v_hi = (float)(tri * (i_hat - ix));
+        return 1;
+    }
+}
+
+static double orc_da(const ctbb_geom *g)
+{
+    /* in-plane deflection, src/generate_system_matrix_ffs.cpp:67 */
+    double D = g->source_detector_distance, R = g->focal_spot_radius;
+    return D * R * sin(g->fan_angle_increment) / (4.0 * (D - R));
+}
+
 /* The ray loop of src/generate_system_matrix.cpp:80-186 (and its FFS twin
  * src/generate_system_matrix_ffs.cpp:104-222): emits (column, q, value) in
  * ascending q. */
 static int orc_ssm_rays(const ctbb_geom *g, emit_fn emit, void *ctx)
 {
     int ffs = (g->zffs != 0 || g->phiffs != 0);
-    double D = g->source_detector_distance, R = g->focal_spot_radius;
-    double da = D * R * sin(g->fan_angle_increment) / (4.0 * (D - R));
+    double da = orc_da(g);
     int nx = (int)g->nx, ny = (int)g->ny;
     for (int i = 0; i < (int)g->n_views_sm; i++) {
         view_geom v;
@@ -165,60 +231,104 @@ static int orc_ssm_rays(const ctbb_geom *g, emit_fn emit, void *ctx)
                 orc_dir(g, &v, ffs, j, k, d);
                 int32_t q = (int32_t)(k + g->n_rows * j
                                       + g->n_rows * g->n_channels * i);
-                double aax = fabs(d[0]), aay = fabs(d[1]);
-                if (aax > aay) {
-                    for (int ix = 0; ix < nx; ix++) {
-                        double x = (ix - g->center_voxel_x) * g->voxel_size_x;
-                        double y_hat = v.s[1] + d[1] / d[0] * (x - v.s[0]);
-                        double j_hat = y_hat / g->voxel_size_y + g->center_voxel_y;
-                        int jy = (int)floor(j_hat);
-                        double z_hat = v.s[2] + d[2] / d[0] * (x - v.s[0]);
-                        double k_hat = z_hat / g->voxel_size_z + g->center_voxel_z;
-                        int kz = (int)floor(k_hat);
-                        if (jy >= 0 && jy <= ny - 2) {
-                            double scale = g->voxel_size_x / aax;
-                            double tri;
-                            if (kz == -1)
-                                tri = scale * (k_hat - kz);
-                            else if (kz == 0)
-                                tri = scale * (1 - (k_hat - kz));
-                            else
-                                continue;
-                            emit(ctx, ix + (int64_t)nx * jy, q,
-                                 (float)(tri * (1 - (j_hat - jy))));
-                            emit(ctx, ix + (int64_t)nx * (jy + 1), q,
-                                 (float)(tri * (j_hat - jy)));
-                        }
-                    }
-                } else {
-                    for (int iy = 0; iy < ny; iy++) {
-                        double y = (iy - g->center_voxel_y) * g->voxel_size_y;
-                        double x_hat = v.s[0] + d[0] / d[1] * (y - v.s[1]);
-                        double i_hat = x_hat / g->voxel_size_x + g->center_voxel_x;
-                        int ix = (int)floor(i_hat);
-                        double z_hat = v.s[2] + d[2] / d[1] * (y - v.s[1]);
-                        double k_hat = z_hat / g->voxel_size_z + g->center_voxel_z;
-                        int kz = (int)floor(k_hat);
-                        if (ix >= 0 && ix <= nx - 2) {
-                            double scale = g->voxel_size_y / aay;
-                            double tri;
-                            if (kz == -1)
-                                tri = scale * (k_hat - kz);
-                            else if (kz == 0)
-                                tri = scale * (1 - (k_hat - kz));
-                            else
-                                continue;
-                            emit(ctx, ix + (int64_t)nx * iy, q,
-                                 (float)(tri * (1 - (i_hat - ix))));
-                            emit(ctx, (ix + 1) + (int64_t)nx * iy, q,
-                                 (float)(tri * (i_hat - ix)));
-                        }
+                int xmajor = fabs(d[0]) > fabs(d[1]);
+                int nmaj = xmajor ? nx : ny;
+                for (int m = 0; m < nmaj; m++) {
+                    int lo;
+                    float a0, a1;
+                    if (!orc_crossing(g, &v, d, xmajor, m, &lo, &a0, &a1))
+                        continue;
+                    if (xmajor) {
+                        emit(ctx, m + (int64_t)nx * lo, q, a0);
+                        emit(ctx, m + (int64_t)nx * (lo + 1), q, a1);
+                    } else {
+                        emit(ctx, lo + (int64_t)nx * m, q, a0);
+                        emit(ctx, (lo + 1) + (int64_t)nx * m, q, a1);
                     }
                 }
             }
         }
     }
     return 0;
+}
+
+/* Entries of one column (ix0, iy0), voxel-driven: every ray of every view
+ * is tested, in ascending q, with the crossing evaluation of the ray loop
+ * above, so the column comes out exactly as orc_ssm_fill writes it.  A ray
+ * crosses the column's line once (x-major: at ix0; y-major: at iy0) and
+ * pushes into it iff the crossed minor index is iy0 or iy0 - 1 (x-major)
+ * resp. ix0 or ix0 - 1.  Channels are screened first with the ray of the
+ * central row, keeping every channel whose crossing lies within two voxels
+ * of the column (the in-plane crossing of a ray does not depend on its row
+ * except through rounding of the normalised direction, far below a voxel);
+ * all rows of the kept channels are then evaluated exactly.
+ * emit == NULL: count only.  Returns the entry count. */
+static int64_t orc_ssm_column(const ctbb_geom *g, int64_t col, int32_t *idx,
+                              float *val)
+{
+    int ffs = (g->zffs != 0 || g->phiffs != 0);
+    double da = orc_da(g);
+    int ix0 = (int)(col % g->nx), iy0 = (int)(col / g->nx);
+    int64_t n = 0;
+    int kc = (int)(g->n_rows / 2);
+    for (int i = 0; i < (int)g->n_views_sm; i++) {
+        view_geom v;
+        orc_view(g, i, ffs, da, &v);
+        for (int j = 0; j < (int)g->n_channels; j++) {
+            double d[3];
+            orc_dir(g, &v, ffs, j, kc, d);
+            int xmajor = fabs(d[0]) > fabs(d[1]);
+            double cross;
+            if (xmajor) {
+                double x = (ix0 - g->center_voxel_x) * g->voxel_size_x;
+                cross = (v.s[1] + d[1] / d[0] * (x - v.s[0])) / g->voxel_size_y
+                        + g->center_voxel_y - iy0;
+            } else {
+                double y = (iy0 - g->center_voxel_y) * g->voxel_size_y;
+                cross = (v.s[0] + d[0] / d[1] * (y - v.s[1])) / g->voxel_size_x
+                        + g->center_voxel_x - ix0;
+            }
+            if (!(cross > -3.0 && cross < 2.0))
+                continue;
+            for (int k = 0; k < (int)g->n_rows; k++) {
+                orc_dir(g, &v, ffs, j, k, d);
+                xmajor = fabs(d[0]) > fabs(d[1]);
+                int lo;
+                float a0, a1;
+                if (!orc_crossing(g, &v, d, xmajor, xmajor ? ix0 : iy0, &lo, &a0, &a1))
+                    continue;
+                int mine = xmajor ? iy0 : ix0;
+                if (lo != mine && lo + 1 != mine)
+                    continue;
+                if (idx) {
+                    idx[n] = (int32_t)(k + g->n_rows * j + g->n_rows * g->n_channels * i);
+                    val[n] = (lo == mine) ? a0 : a1;
+                }
+                n++;
+            }
+        }
+    }
+    return n;
+}
+
+int64_t orc_ssm_cols_count(const ctbb_geom *g, int64_t ncols, const int64_t *cols,
+                           int64_t *cnt)
+{
+    int64_t tot = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+:tot)
+    for (int64_t t = 0; t < ncols; t++) {
+        cnt[t] = orc_ssm_column(g, cols[t], NULL, NULL);
+        tot += cnt[t];
+    }
+    return tot;
+}
+
+void orc_ssm_cols_fill(const ctbb_geom *g, int64_t ncols, const int64_t *cols,
+                       const int64_t *ptr, int32_t *idx, float *val)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t t = 0; t < ncols; t++)
+        (void)orc_ssm_column(g, cols[t], idx + ptr[t], val + ptr[t]);
 }
 
 static void emit_count(void *ctx, int64_t col, int32_t q, float val)
@@ -396,15 +506,18 @@ double orc_edge_preserving(const ctbb_geom *g, int64_t q, double alpha,
 /* ------------------------------------------------------------------------ */
 /* ICD: src/icd_iteration.cpp:149-274 (plus optional weights and the        */
 /* batched z-group order).                                                   */
-/* theta sums and penalty step of voxel (c, k) against the current x, yhat;
- * with ssum (batch surrogate, no reference counterpart) the curvature is
- * sum w a ssum[ray] instead of sum w a^2 */
+/* theta sums and penalty step of voxel (c, k) against the current x, yhat.
+ * alpha squares the FP32 value in FP32 and accumulates in FP64, as
+ * `alpha += nonzeros[m].value * nonzeros[m].value` does
+ * (src/icd_iteration.cpp:208, float * float); beta multiplies in FP64
+ * (:209).  With S (batch surrogate, no reference counterpart) the curvature
+ * is sum_r w_r a_r S[r] over the voxel's data samples r instead. */
 static double orc_voxel_step(const ctbb_geom *g, const int64_t *col_ptr,
                              const int32_t *idx, const float *val,
                              const float *raw, const float *weights, int64_t c,
                              int64_t k, int64_t off, int penalty,
                              double lambda, double delta, const double *x,
-                             const double *yhat, const double *ssum)
+                             const double *yhat, const double *S)
 {
     int64_t data_size = g->readings * g->n_channels * g->n_rows;
     int64_t q = c + g->nx * g->ny * k;
@@ -412,9 +525,15 @@ static double orc_voxel_step(const ctbb_geom *g, const int64_t *col_ptr,
     for (int64_t p = col_ptr[c]; p < col_ptr[c + 1]; p++) {
         int64_t r = idx[p] + off;
         if (r > -1 && r < data_size) {
-            double a = val[p];
+            float af = val[p];
+            double a = af;
             double w = weights ? (double)weights[r] : 1.0;
-            alpha += w * a * (ssum ? ssum[idx[p]] : a);
+            if (S)
+                alpha += w * a * S[r];
+            else if (weights)
+                alpha += w * (double)(af * af);
+            else
+                alpha += (double)(af * af);
             beta += w * a * ((double)raw[r] - yhat[r]);
         }
     }
@@ -442,14 +561,54 @@ static void orc_apply(const ctbb_geom *g, const int64_t *col_ptr,
     }
 }
 
+/* Cost from the maintained yhat (no projection): the orc_cost definition.
+ * par != 0 sums the data term with an OpenMP reduction (the per-step trace,
+ * where the order of the additions does not matter); par == 0 sums in
+ * sample order. */
+static double orc_cost_state(const ctbb_geom *g, const float *raw,
+                             const float *weights, int penalty, double lambda,
+                             double delta, const double *x, const double *yhat,
+                             int par)
+{
+    int64_t data_size = g->readings * g->n_channels * g->n_rows;
+    double data = 0.0;
+#pragma omp parallel for reduction(+:data) if (par)
+    for (int64_t r = 0; r < data_size; r++) {
+        double e = (double)raw[r] - yhat[r];
+        double w = weights ? (double)weights[r] : 1.0;
+        data += w * e * e;
+    }
+    int64_t n = g->nx * g->ny * g->nz, nx = g->nx;
+    int64_t dd[4] = {1, nx - 1, nx, nx + 1};
+    double wd[4] = {1.0, 1.0 / ORC_SQRT2, 1.0, 1.0 / ORC_SQRT2};
+    double pen = 0.0;
+    for (int64_t q = 0; q < n; q++) {
+        for (int t = 0; t < 4; t++) {
+            if (q + dd[t] >= n)
+                continue;
+            double diff = x[q] - x[q + dd[t]];
+            if (penalty == CTBB_PENALTY_QUADRATIC) {
+                pen += wd[t] * diff * diff;
+            } else {
+                double a = fabs(diff);
+                pen += wd[t] * (a - delta * log1p(a / delta));
+            }
+        }
+    }
+    if (penalty == CTBB_PENALTY_QUADRATIC)
+        return 0.5 * data + 0.5 * lambda * pen;
+    return data + lambda * pen;
+}
+
 int64_t orc_icd(const ctbb_geom *g, const int64_t *col_ptr,
                 const int32_t *idx, const float *val, const float *raw,
                 const float *weights, const int64_t *slice_index,
                 int penalty, double lambda, double delta, int64_t spacing,
                 int64_t tile, int n_iter, double *x, double *yhat)
 {
-    return orc_icd_ex(g, col_ptr, idx, val, raw, weights, slice_index, penalty,
-                      lambda, delta, spacing, tile, 0, n_iter, x, yhat);
+    return orc_icd_trace(g, col_ptr, idx, val, raw, weights, slice_index, penalty,
+                         lambda, delta, spacing, tile, 0, n_iter, x, yhat, NULL, 0,
+                         NULL, NULL);
 }
 
 int64_t orc_icd_ex(const ctbb_geom *g, const int64_t *col_ptr,
@@ -459,12 +618,30 @@ int64_t orc_icd_ex(const ctbb_geom *g, const int64_t *col_ptr,
                    int64_t tile, int surrogate, int n_iter, double *x,
                    double *yhat)
 {
+    return orc_icd_trace(g, col_ptr, idx, val, raw, weights, slice_index, penalty,
+                         lambda, delta, spacing, tile, surrogate, n_iter, x, yhat,
+                         NULL, 0, NULL, NULL);
+}
+
+int64_t orc_icd_trace(const ctbb_geom *g, const int64_t *col_ptr,
+                      const int32_t *idx, const float *val, const float *raw,
+                      const float *weights, const int64_t *slice_index,
+                      int penalty, double lambda, double delta, int64_t spacing,
+                      int64_t tile, int surrogate, int n_iter, double *x,
+                      double *yhat, double *step_cost, int64_t max_costs,
+                      int64_t *n_steps, int64_t *shared)
+{
     int64_t updates = 0;
+    int64_t data_size = g->readings * g->n_channels * g->n_rows;
     double fov_limit = (g->acquisition_fov / 2.0) * (g->acquisition_fov / 2.0);
     int64_t *offs = (int64_t *)malloc(sizeof(int64_t) * g->nz);
     for (int64_t k = 0; k < g->nz; k++)
         offs[k] = (slice_index[k] - g->n_views_sm / 2)
                   * g->n_channels * g->n_rows;
+    if (n_steps)
+        *n_steps = 0;
+    if (shared)
+        shared[0] = shared[1] = shared[2] = 0;
     if (spacing <= 1) {
         /* the reference's order: y -> x -> z, each update applied at once */
         for (int n = 0; n < n_iter; n++)
@@ -495,12 +672,28 @@ int64_t orc_icd_ex(const ctbb_geom *g, const int64_t *col_ptr,
     int64_t ntx = (g->nx + tile - 1) / tile, nty = (g->ny + tile - 1) / tile;
     int64_t *bcols = (int64_t *)malloc(sizeof(int64_t) * ntx * nty);
     double *dus = (double *)malloc(sizeof(double) * ntx * nty * g->nz);
-    /* batch surrogate: ssum[i] = sum of the batch's matrix values on SSM ray
-     * i; curvature sum_i w a_ic ssum_i >= sum_i w a_ic^2 majorises the data
-     * term of the whole batch separably (columns of a batch are never
-     * penalty neighbours), so every step decreases the cost */
-    int64_t nray = g->n_views_sm * g->n_channels * g->n_rows;
-    double *ssum = surrogate ? (double *)malloc(sizeof(double) * nray) : NULL;
+    /* batch surrogate: S[r] = sum over every voxel v of the step of its
+     * matrix value on data sample r.  By Cauchy-Schwarz,
+     * (sum_v a_vr d_v)^2 <= S_r sum_v a_vr d_v^2, so the separable quadratic
+     * with curvatures sum_r w_r a_vr S_r majorises the weighted data term of
+     * the whole step -- whatever the columns and slices of the step share --
+     * and the step's voxels are never penalty neighbours (columns >= tile
+     * apart, slices >= s apart), so every step lowers the cost.  Single-column
+     * batches keep the plain curvature unless surrogate == 2 (the step's
+     * samples are disjoint there for a conflict-free spacing, S_r = a_vr). */
+    int single = tile >= g->nx && tile >= g->ny;
+    int use_s = surrogate == 2 || (surrogate && !single);
+    double *S = use_s ? (double *)calloc((size_t)data_size, sizeof(double)) : NULL;
+    int32_t *mk_k = NULL, *mk_c = NULL;
+    if (shared) {
+        mk_k = (int32_t *)malloc(sizeof(int32_t) * data_size);
+        mk_c = (int32_t *)malloc(sizeof(int32_t) * data_size);
+        for (int64_t r = 0; r < data_size; r++)
+            mk_k[r] = -1;
+    }
+    int64_t nst = 0;
+    if (step_cost && max_costs > 0)
+        step_cost[0] = orc_cost_state(g, raw, weights, penalty, lambda, delta, x, yhat, 1);
     for (int n = 0; n < n_iter; n++) {
         for (int64_t ly = 0; ly < tile; ly++)
             for (int64_t lx = 0; lx < tile; lx++) {
@@ -517,20 +710,36 @@ int64_t orc_icd_ex(const ctbb_geom *g, const int64_t *col_ptr,
                     }
                 if (nb == 0)
                     continue;
-                if (ssum) {
-                    memset(ssum, 0, sizeof(double) * nray);
-                    for (int64_t b = 0; b < nb; b++)
-                        for (int64_t p = col_ptr[bcols[b]]; p < col_ptr[bcols[b] + 1]; p++)
-                            ssum[idx[p]] += val[p];
-                }
                 for (int64_t jj = 0; jj < spacing; jj++) {
+                    if (S || shared)
+                        for (int64_t b = 0; b < nb; b++)
+                            for (int64_t k = jj; k < g->nz; k += spacing)
+                                for (int64_t p = col_ptr[bcols[b]]; p < col_ptr[bcols[b] + 1]; p++) {
+                                    int64_t r = idx[p] + offs[k];
+                                    if (!(r > -1 && r < data_size))
+                                        continue;
+                                    if (S)
+                                        S[r] += val[p];
+                                    if (shared) {
+                                        if (mk_k[r] < 0) {
+                                            mk_k[r] = (int32_t)k;
+                                            mk_c[r] = (int32_t)bcols[b];
+                                        } else if (mk_k[r] != k && mk_c[r] != bcols[b]) {
+                                            shared[0]++;
+                                        } else if (mk_k[r] != k) {
+                                            shared[1]++;
+                                        } else if (mk_c[r] != bcols[b]) {
+                                            shared[2]++;
+                                        }
+                                    }
+                                }
                     int64_t t = 0;
                     for (int64_t b = 0; b < nb; b++)
                         for (int64_t k = jj; k < g->nz; k += spacing)
                             dus[t++] = orc_voxel_step(g, col_ptr, idx, val, raw,
                                                       weights, bcols[b], k,
                                                       offs[k], penalty, lambda,
-                                                      delta, x, yhat, ssum);
+                                                      delta, x, yhat, S);
                     t = 0;
                     for (int64_t b = 0; b < nb; b++)
                         for (int64_t k = jj; k < g->nz; k += spacing) {
@@ -538,12 +747,32 @@ int64_t orc_icd_ex(const ctbb_geom *g, const int64_t *col_ptr,
                                       dus[t++], x, yhat);
                             updates++;
                         }
+                    if (S || shared)
+                        for (int64_t b = 0; b < nb; b++)
+                            for (int64_t k = jj; k < g->nz; k += spacing)
+                                for (int64_t p = col_ptr[bcols[b]]; p < col_ptr[bcols[b] + 1]; p++) {
+                                    int64_t r = idx[p] + offs[k];
+                                    if (!(r > -1 && r < data_size))
+                                        continue;
+                                    if (S)
+                                        S[r] = 0.0;
+                                    if (shared)
+                                        mk_k[r] = -1;
+                                }
+                    nst++;
+                    if (step_cost && nst < max_costs)
+                        step_cost[nst] = orc_cost_state(g, raw, weights, penalty,
+                                                        lambda, delta, x, yhat, 1);
                 }
             }
     }
+    if (n_steps)
+        *n_steps = nst;
     free(bcols);
     free(dus);
-    free(ssum);
+    free(S);
+    free(mk_k);
+    free(mk_c);
     free(offs);
     return updates;
 }
@@ -559,33 +788,9 @@ double orc_cost(const ctbb_geom *g, const int64_t *col_ptr,
     int64_t data_size = g->readings * g->n_channels * g->n_rows;
     double *yhat = (double *)malloc(sizeof(double) * data_size);
     orc_forward_project(g, col_ptr, idx, val, slice_index, x, yhat);
-    double data = 0.0;
-    for (int64_t r = 0; r < data_size; r++) {
-        double e = (double)raw[r] - yhat[r];
-        double w = weights ? (double)weights[r] : 1.0;
-        data += w * e * e;
-    }
+    double c = orc_cost_state(g, raw, weights, penalty, lambda, delta, x, yhat, 0);
     free(yhat);
-    int64_t n = g->nx * g->ny * g->nz, nx = g->nx;
-    int64_t dd[4] = {1, nx - 1, nx, nx + 1};
-    double wd[4] = {1.0, 1.0 / ORC_SQRT2, 1.0, 1.0 / ORC_SQRT2};
-    double pen = 0.0;
-    for (int64_t q = 0; q < n; q++) {
-        for (int t = 0; t < 4; t++) {
-            if (q + dd[t] >= n)
-                continue;
-            double diff = x[q] - x[q + dd[t]];
-            if (penalty == CTBB_PENALTY_QUADRATIC) {
-                pen += wd[t] * diff * diff;
-            } else {
-                double a = fabs(diff);
-                pen += wd[t] * (a - delta * log1p(a / delta));
-            }
-        }
-    }
-    if (penalty == CTBB_PENALTY_QUADRATIC)
-        return 0.5 * data + 0.5 * lambda * pen;
-    return data + lambda * pen;
+    return c;
 }
 
 /* ------------------------------------------------------------------------ */

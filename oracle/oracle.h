@@ -25,6 +25,17 @@ int64_t orc_ssm_count(const ctbb_geom *g, int64_t *col_cnt);
 int orc_ssm_fill(const ctbb_geom *g, const int64_t *col_ptr, int32_t *idx,
                  float *val);
 
+/* A3/A4 for a subset of columns, voxel-driven (the same per-crossing
+ * expressions as the ray loop, every ray of every view screened): pass 1
+ * writes cnt[t] = entries of column cols[t] and returns the total; pass 2
+ * fills idx/val at ptr[t] (exclusive scan of cnt) in ascending ray order.
+ * Equal to the matching columns of orc_ssm_fill; cheap for a few columns of
+ * a clinical geometry, where the full ray-driven build is out of reach. */
+int64_t orc_ssm_cols_count(const ctbb_geom *g, int64_t ncols, const int64_t *cols,
+                           int64_t *cnt);
+void orc_ssm_cols_fill(const ctbb_geom *g, int64_t ncols, const int64_t *cols,
+                       const int64_t *ptr, int32_t *idx, float *val);
+
 /* A6: yhat = A x over every slice (FP64, sequential sums). */
 void orc_forward_project(const ctbb_geom *g, const int64_t *col_ptr,
                          const int32_t *idx, const float *val,
@@ -52,15 +63,40 @@ int64_t orc_icd(const ctbb_geom *g, const int64_t *col_ptr,
                 const float *weights, const int64_t *slice_index,
                 int penalty, double lambda, double delta, int64_t spacing,
                 int64_t tile, int n_iter, double *x, double *yhat);
-/* orc_icd with the batch surrogate option (surrogate != 0, batched order
- * only): the curvature of voxel (c, k) is sum_i w a_ic * (sum of the batch's
- * values on SSM ray i) instead of sum_i w a_ic^2. */
+/* orc_icd with the batch surrogate option (surrogate != 0, batched order,
+ * multi-column batches; 2 also for single-column batches): the curvature of
+ * voxel v of a step is sum_r w_r a_vr S_r, S_r = sum over every voxel of the
+ * step (all its columns and slices) of their values on data sample r,
+ * instead of sum_r w_r a_vr^2. */
 int64_t orc_icd_ex(const ctbb_geom *g, const int64_t *col_ptr,
                    const int32_t *idx, const float *val, const float *raw,
                    const float *weights, const int64_t *slice_index,
                    int penalty, double lambda, double delta, int64_t spacing,
                    int64_t tile, int surrogate, int n_iter, double *x,
                    double *yhat);
+
+/* orc_icd_ex that also records step_cost[0] = the cost before the first
+ * step and step_cost[t] = the cost after step t (t < max_costs; batched
+ * order only), the number of steps in *n_steps, and in shared[3] the data
+ * samples reached by two voxels of one step: [0] of different columns and
+ * different slices, [1] of one column and different slices (footprint
+ * conflicts inside a column), [2] of different columns and one slice.
+ * step_cost, n_steps and shared may be NULL. */
+int64_t orc_icd_trace(const ctbb_geom *g, const int64_t *col_ptr,
+                      const int32_t *idx, const float *val, const float *raw,
+                      const float *weights, const int64_t *slice_index,
+                      int penalty, double lambda, double delta, int64_t spacing,
+                      int64_t tile, int surrogate, int n_iter, double *x,
+                      double *yhat, double *step_cost, int64_t max_costs,
+                      int64_t *n_steps, int64_t *shared);
+
+/* CPU baseline: the reference's OpenMP structure (oracle_cpu_ref.c). */
+int64_t orc_icd_ref_omp(const ctbb_geom *g, const int64_t *col_ptr,
+                        const int32_t *idx, const float *val, const float *raw,
+                        const float *weights, const int64_t *slice_index,
+                        int penalty, double lambda, double delta, int n_iter,
+                        int threads, int64_t row_begin, int64_t row_end,
+                        double *x, double *yhat);
 
 /* A9: global cost of x (computes a fresh A x in FP64). */
 double orc_cost(const ctbb_geom *g, const int64_t *col_ptr,

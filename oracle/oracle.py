@@ -56,6 +56,10 @@ def lib() -> C.CDLL:
             "orc_column_overlaps": ([G, _P, _P, _i64, _i64], C.c_int),
             "orc_icd_ref_omp": ([G, _P, _P, _P, _P, _P, _P, C.c_int, _f64, _f64,
                                  C.c_int, C.c_int, _i64, _i64, _P, _P], _i64),
+            "orc_icd_trace": ([G, _P, _P, _P, _P, _P, _P, C.c_int, _f64, _f64, _i64,
+                               _i64, C.c_int, C.c_int, _P, _P, _P, _i64, _P, _P], _i64),
+            "orc_ssm_cols_count": ([G, _i64, _P, _P], _i64),
+            "orc_ssm_cols_fill": ([G, _i64, _P, _P, _P, _P], None),
         }
         for n, (a, r) in sig.items():
             f = getattr(L, n)
@@ -98,6 +102,20 @@ def ssm_build(geom):
     return cp, idx[:nnz], val[:nnz]
 
 
+def ssm_cols(geom, cols):
+    """Entries of the listed columns (voxel-driven restatement; equal to the
+    same columns of ssm_build): (ptr[len(cols)+1], idx, val)."""
+    cols = np.ascontiguousarray(cols, np.int64)
+    cnt = np.zeros(len(cols), np.int64)
+    n = lib().orc_ssm_cols_count(gp(geom), len(cols), _p(cols), _p(cnt))
+    ptr = np.zeros(len(cols) + 1, np.int64)
+    ptr[1:] = np.cumsum(cnt)
+    idx = np.zeros(max(n, 1), np.int32)
+    val = np.zeros(max(n, 1), np.float32)
+    lib().orc_ssm_cols_fill(gp(geom), len(cols), _p(cols), _p(ptr), _p(idx), _p(val))
+    return ptr, idx[:n], val[:n]
+
+
 def forward_project(geom, ssm, slice_index, x):
     cp, idx, val = ssm
     y = np.zeros(geom.data_size, np.float64)
@@ -112,7 +130,8 @@ def icd(geom, ssm, raw, weights, slice_index, penalty, lam, delta, spacing,
     """Run n_iter oracle iterations; returns (x, yhat, updates).
     spacing 0: reference order; spacing >= 2: batched z-group order with the
     given tile (tile >= max(nx, ny): single-column batches), optionally with
-    the batch surrogate curvature."""
+    the batch surrogate curvature (True / 1: multi-column batches; 2: also
+    single-column batches)."""
     cp, idx, val = ssm
     x = np.zeros(geom.nvox, np.float64) if x is None else np.array(x, np.float64)
     if yhat is None:
@@ -123,9 +142,33 @@ def icd(geom, ssm, raw, weights, slice_index, penalty, lam, delta, spacing,
     w = None if weights is None else np.ascontiguousarray(weights, np.float32)
     n = lib().orc_icd_ex(gp(geom), _p(cp), _p(idx), _p(val), _p(raw), _p(w),
                          _p(np.ascontiguousarray(slice_index, np.int64)), penalty, lam,
-                         delta, spacing, tile, int(bool(surrogate)), n_iter, _p(x),
+                         delta, spacing, tile, int(surrogate), n_iter, _p(x),
                          _p(yhat))
     return x, yhat, n
+
+
+def icd_trace(geom, ssm, raw, weights, slice_index, penalty, lam, delta, spacing,
+              n_iter, tile, surrogate=1, max_costs=0, x=None, yhat=None):
+    """Batched oracle sweep that records the cost after every step.
+    Returns (x, yhat, updates, step_costs, shared) with step_costs[0] the
+    cost before the first step and shared = samples reached by two voxels of
+    one step: (different columns and slices, one column / different slices,
+    different columns / one slice)."""
+    cp, idx, val = ssm
+    x = np.zeros(geom.nvox, np.float64) if x is None else np.array(x, np.float64)
+    yhat = forward_project(geom, ssm, slice_index, x) if yhat is None else np.array(yhat)
+    raw = np.ascontiguousarray(raw, np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    costs = np.zeros(max(max_costs, 1), np.float64)
+    nst = np.zeros(1, np.int64)
+    shared = np.zeros(3, np.int64)
+    n = lib().orc_icd_trace(gp(geom), _p(cp), _p(idx), _p(val), _p(raw), _p(w),
+                            _p(np.ascontiguousarray(slice_index, np.int64)), penalty,
+                            lam, delta, spacing, tile, int(surrogate), n_iter, _p(x),
+                            _p(yhat), _p(costs) if max_costs else None, max_costs,
+                            _p(nst), _p(shared))
+    k = min(int(nst[0]) + 1, max_costs)
+    return x, yhat, n, costs[:k], tuple(int(v) for v in shared)
 
 
 def icd_ref_omp(geom, ssm, raw, weights, slice_index, penalty, lam, delta,

@@ -48,9 +48,11 @@ int64_t orc_icd_ref_omp(const ctbb_geom *g, const int64_t *col_ptr,
                     for (int m = 0; m < nnz; m++) {
                         int64_t r = idx[p0 + m] + off;
                         if (r > -1 && r < data_size) {
-                            double a = val[p0 + m];
+                            /* FP32 square, FP64 sums (src/icd_iteration.cpp:208-209) */
+                            float af = val[p0 + m];
+                            double a = af;
                             double w = weights ? (double)weights[r] : 1.0;
-                            alpha += w * a * a;
+                            alpha += weights ? w * (double)(af * af) : (double)(af * af);
                             beta += w * a * ((double)raw[r] - yhat[r]);
                         }
                     }

@@ -82,6 +82,9 @@ def test_driver_end_to_end(gpu, ct, tmp_path):
     dumped = np.fromfile(out / "reconstructions" / "iteration2.rcn", np.float64)
     assert np.array_equal(dumped, x.astype(np.float64))
     assert np.fromfile(out / "reconstructions" / "iteration0.rcn", np.float64).size == g.nvox
+    # zero start: the FP64 sinogram estimate A x0 is zero (src/icd_iteration.cpp:127-133)
+    sino = np.fromfile(out / "reconstructions" / "sino_estimation.rcn", np.float64)
+    assert sino.size == g.data_size and not sino.any()
     fixed = ct.rotate_slices(g, s.tube_angles, s.slice_index, x, 1)
     assert np.array_equal(runs[1], fixed)
 
@@ -137,3 +140,55 @@ def test_driver_fbp_start(gpu, ct, tmp_path):
             assert "Initialized from the GPU FBP image" in p.stdout
         outs.append(np.fromfile(out / "reconstruction.img", np.float32))
     assert not np.array_equal(outs[0], outs[1])
+
+
+def test_driver_initial_volume_from_file(gpu, ct, orc, tmp_path):
+    """initial_recon_path (src/initialize_recon_volume.cpp:51-76): a float32
+    volume in the rotated frame starts the run (its projection is formed
+    first); a missing or truncated file stops the driver with the
+    reference's messages."""
+    from ctbb_icd_amd import configs
+    out = tmp_path / "outf"
+    (out / "reconstructions").mkdir(parents=True)
+    s0 = ct.Scan(configs.write_prm("tiny_ffs", tmp_path / "s0.yaml"))
+    g = s0.geom
+    x0 = (s0.phantom() * 0.9).astype(np.float32)
+    start = tmp_path / "start.img"
+    x0.tofile(start)
+    driver = ROOT / "ctbb_icd_amd" / "ctbb_icd"
+    prm = configs.write_prm("tiny_ffs", tmp_path / "f.yaml", output_dir=str(out),
+                            num_iterations=1, initial_recon_path=str(start))
+    p = subprocess.run([str(driver), str(prm)], capture_output=True, text=True, timeout=120)
+    print(p.stdout[-400:], flush=True)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "Loading initial reconstruction from file" in p.stdout
+    it0 = np.fromfile(out / "reconstructions" / "iteration0.rcn", np.float64)
+    assert np.array_equal(it0, x0.astype(np.float64))
+    # the sinogram estimate is the FP64 projection of the start volume
+    sino = np.fromfile(out / "reconstructions" / "sino_estimation.rcn", np.float64)
+    y = orc.forward_project(g, orc.ssm_build(g), s0.slice_index, x0.astype(np.float64))
+    assert np.abs(sino - y).max() <= 1e-12 * np.abs(y).max()
+    # the same run through the library
+    s = ct.Scan(prm)
+    raw, w = s.synthesize(use_gpu=True)
+    m = ct.SSM.build(g)
+    r = ct.Recon(m, raw, w, s.slice_index, s.opts())
+    r.set_volume(x0)
+    r.iterate(1)
+    fixed = ct.rotate_slices(g, s.tube_angles, s.slice_index, r.volume(), 1)
+    assert np.array_equal(np.fromfile(out / "reconstruction.img", np.float32), fixed)
+    # truncated file
+    short = tmp_path / "short.img"
+    x0[: g.nvox // 2].tofile(short)
+    prm2 = configs.write_prm("tiny_ffs", tmp_path / "f2.yaml", output_dir=str(out),
+                             num_iterations=1, initial_recon_path=str(short))
+    p = subprocess.run([str(driver), str(prm2)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 1
+    assert "File was not read properly." in p.stdout
+    assert "Could not initialize reconstruction volume from given file. Exiting" in p.stdout
+    # missing file
+    prm3 = configs.write_prm("tiny_ffs", tmp_path / "f3.yaml", output_dir=str(out),
+                             num_iterations=1, initial_recon_path=str(tmp_path / "none.img"))
+    p = subprocess.run([str(driver), str(prm3)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 1
+    assert "Specified initial reconstruction file not found. Exiting" in p.stdout

@@ -99,17 +99,17 @@ def test_penalty_steps(gpu, make_scan, orc):
             else:
                 ref = orc.edge_preserving(g, int(q[i]), alpha[i], beta[i], 0.7, 0.02, x)
             assert du[i] == ref or abs(du[i] - ref) <= 1e-14 * max(1.0, abs(ref)), (pen, i)
-        # the sweep's solver (one wavefront, 65-section) on the FP32 volume:
-        # within the reference's 1e-8 tolerance of the reference's answer
+        # the sweep's instantiation (the volume held in FP32): the reference's
+        # step on the FP32-rounded volume
         x32 = x.astype(np.float32).astype(np.float64)
-        du_w = gpu.penalty_steps(g, o, x, q, alpha, beta, solver=1)
+        du_w = gpu.penalty_steps(g, o, x, q, alpha, beta, volume_fp32=True)
         for i in range(500):
             if pen == 0:
                 ref = orc.quadratic(g, int(q[i]), alpha[i], beta[i], 0.7, x32)
-                assert abs(du_w[i] - ref) <= 1e-14 * max(1.0, abs(ref)), (pen, i)
             else:
                 ref = orc.edge_preserving(g, int(q[i]), alpha[i], beta[i], 0.7, 0.02, x32)
-                assert abs(du_w[i] - ref) <= 2e-8, (pen, i, du_w[i], ref)
+            assert du_w[i] == ref or abs(du_w[i] - ref) <= 1e-14 * max(1.0, abs(ref)), \
+                (pen, i, du_w[i], ref)
 
 
 @pytest.mark.parametrize("name", ["tiny", "tiny_ffs", "config0"])
