@@ -131,6 +131,7 @@ def lib() -> C.CDLL:
         "ctbb_recon_cost_running": ([P, P], C.c_int),
         "ctbb_recon_get_error": ([P, P], C.c_int),
         "ctbb_cost": ([P, P, P, P, C.POINTER(IcdOpts), P, P], C.c_int),
+        "ctbb_cost_noalloc": ([P, P, P, P, C.POINTER(IcdOpts), P, P, P], C.c_int),
         "ctbb_rotate_slices": ([C.POINTER(Geom), P, P, P, C.c_int], C.c_int),
         "ctbb_penalty_steps": ([C.POINTER(Geom), C.POINTER(IcdOpts), C.c_int, P, _i64, P, P, P, P], C.c_int),
         "ctbb_unet_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,

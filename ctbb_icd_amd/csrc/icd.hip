@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 
 using namespace ctbb;
@@ -2242,6 +2243,146 @@ extern "C" int ctbb_cost(const ctbb_ssm *m, const float *raw, const float *weigh
         *cost = 0.5 * data + 0.5 * opts->lambda * pen;
     else
         *cost = data + opts->lambda * pen;
+    return CTBB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free cost (ctbb_cost_noalloc).  Its scratch -- the slice and
+// view tables of the projection, the per-workgroup partials of the data and
+// penalty terms and the result slot -- lives in one file-scope device array
+// per GPU, addressed through the pointer hipGetSymbolAddress returns on that
+// GPU (the array's name is a host-side stand-in, never a device address).
+// Every call on one GPU shares that array: calls that overlap on one GPU
+// race; consecutive calls on one stream are ordered.
+
+constexpr size_t kNoAllocScratch = size_t(8) << 20;   // bytes, per GPU
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Device address of this GPU's scratch array, looked up once per GPU (with
+// that GPU current) on the first call there.
+static int noalloc_scratch(void **out)
+{
+    int dev = 0;
+    CTBB_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(CTBB_E_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(CTBB_E_HIP, std::string("scratch symbol address: ") +
+                                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return CTBB_OK;
+}
+
+// Slice view offsets and, per data view, the slices whose view window holds
+// it (the device form of view_slice_ranges).  One thread per data view;
+// thread v < nz also writes si_half[v].
+__global__ void noalloc_tables_kernel(const int64_t *slice_index, int64_t nz,
+                                      int64_t readings, int64_t nsm, int64_t nffs,
+                                      int ffs, int64_t *si_half, int32_t *klo,
+                                      int32_t *khi)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    auto half = [&](int64_t k) {
+        int64_t c = slice_index[k];
+        if (ffs)
+            c -= c % nffs;
+        return c - nsm / 2;
+    };
+    if (v < nz)
+        si_half[v] = half(v);
+    if (v >= readings)
+        return;
+    int32_t lo = -1, hi = -1;
+    for (int64_t k = 0; k < nz; k++) {
+        const int64_t i = v - half(k);
+        if (i >= 0 && i < nsm) {
+            if (lo < 0)
+                lo = (int32_t)k;
+            hi = (int32_t)k + 1;
+        }
+    }
+    klo[v] = lo < 0 ? 0 : lo;
+    khi[v] = lo < 0 ? 0 : hi;
+}
+
+// The two partial arrays summed in block order by one thread (the order of
+// sum_partials on the host), combined as in ctbb_cost.
+__global__ void noalloc_finish_kernel(const double *pdata, int64_t ndata,
+                                      const double *ppen, int64_t npen, int penalty,
+                                      double lambda, double *cost)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    double data = 0.0, pen = 0.0;
+    for (int64_t i = 0; i < ndata; i++)
+        data += pdata[i];
+    for (int64_t i = 0; i < npen; i++)
+        pen += ppen[i];
+    *cost = penalty == CTBB_PENALTY_QUADRATIC ? 0.5 * data + 0.5 * lambda * pen
+                                              : data + lambda * pen;
+}
+
+extern "C" int ctbb_cost_noalloc(const ctbb_ssm *m, const float *raw,
+                                 const float *weights, const int64_t *slice_index,
+                                 const ctbb_icd_opts *opts, const float *volume,
+                                 double *cost, void *stream)
+{
+    if (!m || !raw || !slice_index || !opts || !volume || !cost)
+        return fail(CTBB_E_ARG, "null argument");
+    if (opts->penalty != CTBB_PENALTY_QUADRATIC &&
+        opts->penalty != CTBB_PENALTY_EDGE_PRESERVING)
+        return fail(CTBB_E_ARG, "unknown penalty (valid: quadratic, edge-preserving)");
+    const ctbb_geom *g = &m->g;
+    const int64_t nvox = g->nx * g->ny * g->nz;
+    const int64_t nproj = project_grid(g), npen = red_grid(nvox);
+    // scratch layout: si_half [nz] | klo [readings] | khi [readings] |
+    // data partials [nproj] | penalty partials [npen]
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t o_klo = up(8 * (size_t)g->nz);
+    const size_t o_khi = o_klo + up(4 * (size_t)g->readings);
+    const size_t o_pd = o_khi + up(4 * (size_t)g->readings);
+    const size_t o_pp = o_pd + up(8 * (size_t)nproj);
+    const size_t need = o_pp + up(8 * (size_t)npen);
+    if (need > kNoAllocScratch)
+        return fail(CTBB_E_ARG, "geometry needs " + std::to_string(need) +
+                                    " bytes of scratch; the allocation-free cost has " +
+                                    std::to_string(kNoAllocScratch) +
+                                    " (readings x channel groups too large)");
+    if (!m->rm_vc)
+        return fail(CTBB_E_STATE, "the matrix has no ray-major copy yet: call ctbb_cost, "
+                                  "ctbb_forward_project or ctbb_recon_create on it once "
+                                  "(they build it) before the allocation-free cost");
+    void *sp = nullptr;
+    CTBB_TRY(noalloc_scratch(&sp));
+    unsigned char *base = static_cast<unsigned char *>(sp);
+    int64_t *d_sih = reinterpret_cast<int64_t *>(base);
+    int32_t *d_klo = reinterpret_cast<int32_t *>(base + o_klo);
+    int32_t *d_khi = reinterpret_cast<int32_t *>(base + o_khi);
+    double *d_pd = reinterpret_cast<double *>(base + o_pd);
+    double *d_pp = reinterpret_cast<double *>(base + o_pp);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t nt = std::max(g->readings, g->nz);
+    hipLaunchKernelGGL(noalloc_tables_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256),
+                       0, st, slice_index, g->nz, g->readings, g->n_views_sm, nffs_of(g),
+                       (g->zffs || g->phiffs) ? 1 : 0, d_sih, d_klo, d_khi);
+    CTBB_HIP(hipGetLastError());
+    // every partial slot is written by its workgroup: no clearing needed
+    CTBB_TRY((launch_project<float, 1>(m, volume, d_sih, d_klo, d_khi, nullptr, raw, weights,
+                                       nullptr, nullptr, d_pd, st)));
+    hipLaunchKernelGGL(penalty_cost_kernel, dim3((unsigned)npen), dim3(kRedBlock), 0, st,
+                       volume, g->nx, nvox, opts->penalty, opts->delta, d_pp);
+    hipLaunchKernelGGL(noalloc_finish_kernel, dim3(1), dim3(64), 0, st, d_pd, nproj, d_pp,
+                       npen, opts->penalty, opts->lambda, cost);
+    CTBB_HIP(hipGetLastError());
     return CTBB_OK;
 }
 

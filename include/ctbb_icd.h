@@ -265,6 +265,26 @@ int ctbb_cost(const ctbb_ssm *m, const float *raw, const float *weights,
               const int64_t *slice_index, const ctbb_icd_opts *opts,
               const float *volume, double *cost);
 
+/* Allocation-free ctbb_cost.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same cost as
+ * ctbb_cost, bit for bit, but every pointer except m and opts is DEVICE
+ * memory: raw and weights (or NULL) [readings*n_channels*n_rows], slice_index
+ * [nz] (int64), volume [nx*ny*nz] (FP32, rotated frame), and the result
+ * *cost (one FP64 value, written by the last kernel).  The work is enqueued
+ * on `stream` (a hipStream_t; NULL = the null stream) and the call returns
+ * without waiting.  Scratch comes from one static 8 MiB device array per
+ * GPU: geometries needing more (about readings * ceil(n_channels/32) > 1e6)
+ * are rejected with CTBB_E_ARG; the array is shared by every call on that
+ * GPU and keeps the last call's contents, so calls that overlap on one GPU
+ * (other streams, other threads) race -- order them.  The matrix needs its
+ * ray-major copy, which ctbb_cost, ctbb_forward_project and
+ * ctbb_recon_create build (allocating) on first use: before that this call
+ * fails with CTBB_E_STATE.  It calls no hipMalloc or hipFree. */
+int ctbb_cost_noalloc(const ctbb_ssm *m, const float *raw, const float *weights,
+                      const int64_t *slice_index, const ctbb_icd_opts *opts,
+                      const float *volume, double *cost, void *stream);
+
 /* ---- slice rotation -----------------------------------------------------
  * Replaces rotate_slices_rotating2fixed / rotate_slices_fixed2rotating
  * (src/rotate_slices.h:25,29; src/rotate_slices.cpp:48-154 with interp2 of
