@@ -60,7 +60,7 @@ class RunParams(C.Structure):
         ("z_end", _f64), ("beam_width", _f64), ("beam_width_at_roi_edge", _f64),
         ("recon_fov", _f64), ("synthetic", _i32), ("use_weights", _i32),
         ("ssm_fp16", _i32), ("neighbours", _i32), ("resync_every", _i32),
-        ("reserved", _i32), ("group_spacing", _i64),
+        ("state_fp64", _i32), ("group_spacing", _i64),
     ]
 
 
@@ -68,7 +68,7 @@ class IcdOpts(C.Structure):
     _fields_ = [
         ("penalty", _i32), ("neighbours", _i32), ("lambda_", _f64),
         ("delta", _f64), ("group_spacing", _i64), ("resync_every", _i32),
-        ("tile", _i32), ("plain_curvature", _i32),
+        ("tile", _i32), ("plain_curvature", _i32), ("state_fp64", _i32),
     ]
 
 
@@ -125,6 +125,8 @@ def lib() -> C.CDLL:
         "ctbb_recon_free": ([P], C.c_int),
         "ctbb_recon_set_volume": ([P, P], C.c_int),
         "ctbb_recon_get_volume": ([P, P], C.c_int),
+        "ctbb_recon_set_volume64": ([P, P], C.c_int),
+        "ctbb_recon_get_volume64": ([P, P], C.c_int),
         "ctbb_recon_iterate": ([P, C.c_int, C.POINTER(IcdStats)], C.c_int),
         "ctbb_recon_resync": ([P], C.c_int),
         "ctbb_recon_cost": ([P, P], C.c_int),
@@ -234,6 +236,7 @@ class Scan:
         o.resync_every = kw.get("resync_every", self.run.resync_every)
         o.tile = kw.get("tile", self.tile)
         o.plain_curvature = kw.get("plain_curvature", self.plain_curvature)
+        o.state_fp64 = kw.get("state_fp64", self.run.state_fp64)
         return o
 
 
@@ -374,6 +377,16 @@ class Recon:
         v = np.zeros(self.geom.nvox, np.float32)
         check(lib().ctbb_recon_get_volume(self.h, _p(v)))
         return v
+
+    def volume64(self) -> np.ndarray:
+        v = np.zeros(self.geom.nvox, np.float64)
+        check(lib().ctbb_recon_get_volume64(self.h, _p(v)))
+        return v
+
+    def set_volume64(self, vol: np.ndarray) -> None:
+        vol = np.ascontiguousarray(vol, np.float64)
+        _need(vol, self.geom.nvox, "volume")
+        check(lib().ctbb_recon_set_volume64(self.h, _p(vol)))
 
     def error(self) -> np.ndarray:
         e = np.zeros(self.geom.data_size, np.float32)

@@ -102,21 +102,25 @@ struct ctbb_recon {
     int64_t nvox = 0;
     float *y = nullptr;         // measured line integrals
     float *w = nullptr;         // weights or null
-    float *e = nullptr;         // running error sinogram y - A x (FP32)
-    float *x = nullptr;         // volume, rotated frame
+    // sweep state, FP32 or FP64 (state_fp64): e, ew, x, du, the partials,
+    // the surrogate sums and the +-1 scatter weights hold `sb`-byte values
+    bool fp64 = false;
+    size_t sb = 4;
+    void *e = nullptr;          // running error sinogram y - A x
+    void *x = nullptr;          // volume, rotated frame
     int64_t *slice_off = nullptr;  // [nz] sample offsets
     int32_t *cols = nullptr;    // FOV columns, grouped by batch
     int64_t *batch_ptr = nullptr;  // [nbatch+1] offsets into cols
-    float *ew = nullptr;        // weighted error w e (weighted recons)
-    float *du = nullptr;        // per-step voxel updates
+    void *ew = nullptr;         // weighted error w e (weighted recons)
+    void *du = nullptr;         // per-step voxel updates
     double *th2 = nullptr;      // theta2 (batch surrogate curvature) per voxel
     bool th2_valid = false;     //   [nvox], iteration-invariant, first sweep
     // batch surrogate: S[r] = sum over the voxels of one step (every column
     // of the batch, every slice of the group) of their matrix values on data
     // sample r; zero between steps (first sweep only)
-    float *ssum = nullptr;        // [data_size] or null (plain curvature)
-    float *du_m1 = nullptr;       // [(max_batch+8)*nz] all -1: scatter S += a
-    float *du_p1 = nullptr;       // [(max_batch+8)*nz] all +1: scatter S -= a
+    void *ssum = nullptr;         // [data_size] or null (plain curvature)
+    void *du_m1 = nullptr;        // [(max_batch+8)*nz] all -1: scatter S += a
+    void *du_p1 = nullptr;        // [(max_batch+8)*nz] all +1: scatter S -= a
     // view-blocked batch copy of the matrix (gather): per batch b and view
     // block ib (kViewBlock views) the columns ranked by detector position,
     // in chunks of 64 ranks; a chunk's entries ordered (view in block, rank)
@@ -129,8 +133,8 @@ struct ctbb_recon {
     std::vector<int64_t> h_wbase; // [nbatch+1] first chunk of each batch
     int64_t nib = 0;              // view blocks
     std::vector<int64_t> view_stride;   // [spacing] scatter dispatch stride
-    float *part1 = nullptr;       // per-step partials [ng][nib][max_batch]
-    float *part2 = nullptr;       //   (theta1, theta2)
+    void *part1 = nullptr;        // per-step partials [ng][nib][max_batch]
+    void *part2 = nullptr;        //   (theta1, theta2)
     int64_t ncols_fov = 0;
     int64_t nbatch = 0;
     int64_t max_batch = 0;

@@ -174,17 +174,20 @@ __global__ void penalty_steps_kernel(int penalty, double lambda, double delta,
 constexpr int kSweepBlock = 256;
 constexpr int kWaves = kSweepBlock / 64;
 
+// The sweep state (error sinograms, volume, updates, partials, surrogate
+// sums) is FP32 or FP64; the kernels take its type T as a template argument
+// and the argument blocks carry the state untyped.
 struct SweepArgs {
     const int64_t *col_ptr;
     const int32_t *idx;
     const float *val;
     const __half *val16;
     const float *col_scale;
-    float *e;                 // error sinogram y - A x
-    float *ew;                // weighted error w (y - A x) (== e unweighted)
+    void *e;                  // T: error sinogram y - A x
+    void *ew;                 // T: weighted error w (y - A x) (== e unweighted)
     const float *w;           // weights or null
-    float *x;
-    float *du;                // [ng*nc] updates of the step
+    void *x;                  // T: volume
+    void *du;                 // T [ng*nc]: updates of the step
     double *th2;              // [nvox] theta2 per voxel (cached)
     const int64_t *slice_off;
     const int32_t *cols;      // batch columns, concatenated
@@ -193,21 +196,6 @@ struct SweepArgs {
     int penalty;
     double lambda, delta;
 };
-
-__device__ __forceinline__ float wave_sum(float v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-template <bool FP16>
-__device__ __forceinline__ float ssm_value(const SweepArgs &a, int64_t p,
-                                           float scale)
-{
-    return FP16 ? __half2float(a.val16[p]) * scale : a.val[p];
-}
 
 // Inclusive sum over runs of equal keys across the lanes of a wavefront:
 // bit l of `head` is set where lane l starts a run (lane 0 always does), and
@@ -303,11 +291,11 @@ struct GatherArgs {
     int64_t base;             // nib * (first FOV index of the batch)
     int64_t wbase;            // first chunk of the batch
     int64_t nw;               // 64-rank chunks of the batch
-    float *part1, *part2;     // [ng][nib][nc]
+    void *part1, *part2;      // T [ng][nib][nc]
     int64_t nib;
     const int64_t *si_half;   // [nz] central view - n_sm/2
-    const float *ssum;        // batch surrogate: sum of the step's values per
-                              //   data sample (null: sum w a^2)
+    const void *ssum;         // T: batch surrogate, sum of the step's values
+                              //   per data sample (null: sum w a^2)
     int64_t nsc;              // workgroups per view block
     int64_t nsm, readings;
 };
@@ -329,14 +317,14 @@ __device__ __forceinline__ float gb_value(const GatherArgs &g, int64_t p)
     return FP16 ? __half2float(g.val16[p]) : g.val[p];
 }
 
-template <bool FP16, bool THETA2>
+template <typename T, bool FP16, bool THETA2>
 __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                                                                  StepArgs sa,
                                                                  GatherArgs g)
 {
     constexpr int kAcc = 64 * kViewBlock;
     constexpr int M = THETA2 ? kGatherMembersFirst : kGatherMembers;
-    __shared__ float accm[THETA2 ? 2 * M : M][kWaves * kAcc];
+    __shared__ T accm[THETA2 ? 2 * M : M][kWaves * kAcc];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t nwg = g.nib * g.nsc;
     const int64_t per_xcd = (nwg + kXcd - 1) / kXcd;
@@ -352,7 +340,8 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
     const int64_t rk = w * 64 + lane;
     const bool own = rk < sa.nc;
     const int64_t slot = own ? (int64_t)g.perm[g.base + ib * sa.nc + rk] : 0;
-    const float *__restrict__ ew = a.ew;
+    const T *__restrict__ ew = static_cast<const T *>(a.ew);
+    const T *__restrict__ ssum = static_cast<const T *>(g.ssum);
     const float *__restrict__ wt = a.w;
     const uint32_t ds = (uint32_t)a.data_size;
     // M members of the slice group per pass over the chunk's entries, each
@@ -381,7 +370,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
         for (int m = 0; m < (THETA2 ? 2 * M : M); m++)
 #pragma unroll
             for (int q = 0; q < kViewBlock; q++)
-                accm[m][wave * kAcc + q * 64 + lane] = 0.0f;
+                accm[m][wave * kAcc + q * 64 + lane] = T(0);
         for (int64_t p0 = lo; p0 < hi; p0 += 64 * kGatherUnroll) {
             int32_t iv[kGatherUnroll], kv[kGatherUnroll];
             float av[kGatherUnroll];
@@ -393,8 +382,8 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                 av[u] = ok ? gb_value<FP16>(g, p) : 0.0f;
                 kv[u] = ok ? (int32_t)g.key[p] : -1;
             }
-            float ev[M][kGatherUnroll], mv[M][kGatherUnroll], wv[M][kGatherUnroll];
-            float sv[M][kGatherUnroll];
+            T ev[M][kGatherUnroll], sv[M][kGatherUnroll];
+            float mv[M][kGatherUnroll], wv[M][kGatherUnroll];
 #pragma unroll
             for (int m = 0; m < M; m++)
 #pragma unroll
@@ -405,20 +394,24 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                     mv[m][u] = in ? av[u] : 0.0f;
                     ev[m][u] = ew[in ? r : 0];
                     wv[m][u] = (THETA2 && wt) ? wt[in ? r : 0] : 1.0f;
-                    // surrogate: the step's summed values on this sample
-                    sv[m][u] = THETA2 ? (g.ssum ? g.ssum[in ? r : 0] : av[u]) : 0.0f;
+                    // curvature factor: the step's summed values on this
+                    // sample (surrogate), or a itself (a^2 squared in FP32,
+                    // as src/icd_iteration.cpp:208)
+                    if (THETA2)
+                        sv[m][u] = ssum ? (T)mv[m][u] * ssum[in ? r : 0]
+                                        : (T)(mv[m][u] * av[u]);
                 }
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; u++) {
                 if (p0 + u * 64 >= hi)
                     break;
                 const uint64_t head = run_heads(kv[u], lane);
-                float t1[M], t2[M];
+                T t1[M], t2[M];
 #pragma unroll
                 for (int m = 0; m < M; m++) {
-                    t1[m] = run_sum(mv[m][u] * ev[m][u], head, lane);
+                    t1[m] = run_sum((T)mv[m][u] * ev[m][u], head, lane);
                     if (THETA2)
-                        t2[m] = run_sum((wv[m][u] * mv[m][u]) * sv[m][u], head, lane);
+                        t2[m] = run_sum((T)wv[m][u] * sv[m][u], head, lane);
                 }
                 if (kv[u] >= 0 && run_end(head, lane)) {
 #pragma unroll
@@ -432,7 +425,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
         }
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            float s1 = 0.0f, s2 = 0.0f;
+            T s1 = T(0), s2 = T(0);
 #pragma unroll
             for (int q = 0; q < kViewBlock; q++) {
                 s1 += accm[m][wave * kAcc + q * 64 + lane];
@@ -441,9 +434,9 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
             }
             if (own && has[m]) {
                 const int64_t o = (vm[m] * g.nib + ib) * sa.nc + slot;
-                g.part1[o] = s1;
+                static_cast<T *>(g.part1)[o] = s1;
                 if (THETA2)
-                    g.part2[o] = s2;
+                    static_cast<T *>(g.part2)[o] = s2;
             }
         }
     }
@@ -455,11 +448,14 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
 // thread per voxel takes the penalty step with the reference's solver in
 // FP64, clamps to x >= 0 and writes x and the update du.  A workgroup covers
 // 64 slots of one group member.
-template <bool FP16, bool THETA2>
+template <typename T, bool FP16, bool THETA2>
 __global__ __launch_bounds__(kSweepBlock) void icd_solve_kernel(SweepArgs a,
                                                                 StepArgs sa,
                                                                 GatherArgs g)
 {
+    const T *part1 = static_cast<const T *>(g.part1);
+    const T *part2 = static_cast<const T *>(g.part2);
+    T *x = static_cast<T *>(a.x);
     __shared__ double s1[kWaves][64], s2[kWaves][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t nch64 = (sa.nc + 63) / 64;
@@ -478,11 +474,11 @@ __global__ __launch_bounds__(kSweepBlock) void icd_solve_kernel(SweepArgs a,
         const int64_t o = v * g.nib * sa.nc + s;
         int64_t ib = ib0 + wave;
         for (; ib + 3 * kWaves < ib1; ib += 4 * kWaves) {
-            float q1[4], q2[4];
+            T q1[4], q2[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                q1[u] = g.part1[o + (ib + u * kWaves) * sa.nc];
-                q2[u] = THETA2 ? g.part2[o + (ib + u * kWaves) * sa.nc] : 0.0f;
+                q1[u] = part1[o + (ib + u * kWaves) * sa.nc];
+                q2[u] = THETA2 ? part2[o + (ib + u * kWaves) * sa.nc] : T(0);
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -491,9 +487,9 @@ __global__ __launch_bounds__(kSweepBlock) void icd_solve_kernel(SweepArgs a,
             }
         }
         for (; ib < ib1; ib += kWaves) {
-            t1 += (double)g.part1[o + ib * sa.nc];
+            t1 += (double)part1[o + ib * sa.nc];
             if (THETA2)
-                t2 += (double)g.part2[o + ib * sa.nc];
+                t2 += (double)part2[o + ib * sa.nc];
         }
     }
     s1[wave][lane] = t1;
@@ -519,13 +515,13 @@ __global__ __launch_bounds__(kSweepBlock) void icd_solve_kernel(SweepArgs a,
         a.th2[q] = th2;
     else
         th2 = a.th2[q];
-    double du = penalty_step(a.penalty, q, th2, th1, a.lambda, a.delta, a.x, a.nx,
+    double du = penalty_step(a.penalty, q, th2, th1, a.lambda, a.delta, x, a.nx,
                              a.nvox);
-    const double xq = (double)a.x[q];
+    const double xq = (double)x[q];
     if (du + xq < 0)
         du = -xq;
-    a.x[q] = (float)(xq + du);
-    a.du[v * sa.nc + s] = (float)du;
+    x[q] = (T)(xq + du);
+    static_cast<T *>(a.du)[v * sa.nc + s] = (T)du;
 }
 
 // Phase B of a step: sample-driven scatter.  Each workgroup owns the
@@ -576,8 +572,8 @@ struct BtArgs {
 
 // One 64-entry chunk of an item: reduce runs of equal samples and add them
 // into the view's tile (acc: rows of nr samples per channel from c0).
-__device__ __forceinline__ void scatter_chunk(uint32_t e0, float d, bool ok, int lane,
-                                              float *acc)
+template <typename T>
+__device__ __forceinline__ void scatter_chunk(uint32_t e0, T d, bool ok, int lane, T *acc)
 {
     const int key = ok ? (int)bt_lkey(e0) : -1;
     const uint64_t head = run_heads(key, lane);
@@ -596,11 +592,14 @@ struct ScatterOrder {
     int64_t vlo, vhi, nq, ncg, P, rows;
 };
 
-// LDS: kScatterViews * kScatterCh * nr floats (dynamic).
+// LDS: kScatterViews * kScatterCh * nr values of T (dynamic).
+template <typename T>
 __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
     SweepArgs a, StepArgs sa, BtArgs bt, int64_t b, ScatterOrder so)
 {
-    extern __shared__ float acc[];
+    extern __shared__ __align__(8) unsigned char acc_raw[];
+    T *acc = reinterpret_cast<T *>(acc_raw);
+    const T *du = static_cast<const T *>(a.du);
     __shared__ int any_work;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t nwg = so.P * so.rows * so.ncg;
@@ -627,7 +626,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
     if (tid == 0)
         any_work = 0;
     for (int i = tid; i < kScatterViews * tile; i += kSweepBlock)
-        acc[i] = 0.0f;
+        acc[i] = T(0);
     __syncthreads();
     // items: lane l of a round covers view u = l / kItemLanes and member
     // vb + l % kItemLanes
@@ -686,12 +685,12 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
                 ev[u] = ok[u] ? ent[p] : 0u;
                 av[u] = ok[u] ? val[p] : 0.0f;
             }
-            float dv[kItemUnroll];
+            T dv[kItemUnroll];
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
                 const int32_t vm = (int32_t)vb + (it[u] < 0 ? 0 : (it[u] % kItemLanes));
-                dv[u] = ok[u] ? a.du[vm * (int32_t)sa.nc + (int32_t)bt_slot(ev[u])] * av[u]
-                              : 0.0f;
+                dv[u] = ok[u] ? du[vm * (int32_t)sa.nc + (int32_t)bt_slot(ev[u])] * (T)av[u]
+                              : T(0);
             }
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
@@ -706,22 +705,22 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
         return;
     for (int64_t u = 0; u < nview; u++) {
         const int64_t r0 = ((view0 + u) * bt.nch + c0) * bt.nr;
-        const float *tacc = acc + u * tile;
-        float *e = a.e + r0;
+        const T *tacc = acc + u * tile;
+        T *e = static_cast<T *>(a.e) + r0;
         if (a.w) {
-            float *ew = a.ew + r0;
+            T *ew = static_cast<T *>(a.ew) + r0;
             const float *w = a.w + r0;
             for (int i = tid; i < tile; i += kSweepBlock) {
-                const float d = tacc[i];
-                if (d != 0.0f) {
+                const T d = tacc[i];
+                if (d != T(0)) {
                     e[i] = e[i] - d;
-                    ew[i] = ew[i] - w[i] * d;
+                    ew[i] = ew[i] - (T)w[i] * d;
                 }
             }
         } else {
             for (int i = tid; i < tile; i += kSweepBlock) {
-                const float d = tacc[i];
-                if (d != 0.0f)
+                const T d = tacc[i];
+                if (d != T(0))
                     e[i] = e[i] - d;
             }
         }
@@ -915,17 +914,28 @@ __device__ double block_sum_f64(double v, double *sh)
     return s;
 }
 
-// ew = w e (FP32 of the FP64 product).
+// ew = w e (the FP64 product, rounded to the state type T).
+template <typename T>
 __global__ __launch_bounds__(kRedBlock) void weighted_error_kernel(
-    const float *e, const float *w, int64_t n, float *ew)
+    const T *e, const float *w, int64_t n, T *ew)
 {
     for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kRedBlock)
-        ew[i] = (float)((double)w[i] * (double)e[i]);
+        ew[i] = (T)((double)w[i] * (double)e[i]);
 }
 
-// Data part of the cost from the kept FP32 error sinogram: sum w e^2, FP64.
-__global__ __launch_bounds__(kRedBlock) void error_sq_kernel(const float *e, const float *w,
+// out = (T) in: the measured sinogram as the starting error sinogram.
+template <typename T>
+__global__ __launch_bounds__(kRedBlock) void widen_kernel(const float *in, int64_t n, T *out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kRedBlock)
+        out[i] = (T)in[i];
+}
+
+// Data part of the cost from the kept error sinogram: sum w e^2, FP64.
+template <typename T>
+__global__ __launch_bounds__(kRedBlock) void error_sq_kernel(const T *e, const float *w,
                                                              int64_t n, double *partials)
 {
     __shared__ double sh[kRedBlock / 64];
@@ -941,8 +951,9 @@ __global__ __launch_bounds__(kRedBlock) void error_sq_kernel(const float *e, con
 }
 
 // Penalty part of the cost: sum over pairs (q, q+d), d in {1, nx-1, nx, nx+1}.
+template <typename T>
 __global__ __launch_bounds__(kRedBlock) void penalty_cost_kernel(
-    const float *x, int64_t nx, int64_t n, int penalty, double delta,
+    const T *x, int64_t nx, int64_t n, int penalty, double delta,
     double *partials)
 {
     __shared__ double sh[kRedBlock / 64];
@@ -1115,12 +1126,12 @@ __global__ void rm_group_kernel(const int64_t *rptr, int64_t ngroups, int64_t nr
 // one partial of w (y - A x)^2 per workgroup.  LDS: kScatterCh * nr doubles.
 constexpr int kProjUnroll = 4;
 
-template <typename X, int MODE>
+template <typename X, typename E, int MODE>
 __global__ __launch_bounds__(kSweepBlock) void project_gather_kernel(
     const int64_t *vc_ptr, const uint32_t *rent, const float *rval, const X *x,
     const int64_t *si_half, const int32_t *k_lo, const int32_t *k_hi,
     int64_t nsm, int64_t nch, int64_t nr, int64_t nxy, int64_t ncg,
-    double *yhat, const float *y, const float *w, float *e, float *ew,
+    double *yhat, const float *y, const float *w, E *e, E *ew,
     double *partials)
 {
     extern __shared__ double pacc[];
@@ -1181,9 +1192,9 @@ __global__ __launch_bounds__(kSweepBlock) void project_gather_kernel(
             const double d = (double)y[r0 + i] - acc;
             const double wi = w ? (double)w[r0 + i] : 1.0;
             if (e)
-                e[r0 + i] = (float)d;
+                e[r0 + i] = (E)d;
             if (ew)
-                ew[r0 + i] = (float)(wi * d);
+                ew[r0 + i] = (E)(wi * d);
             part += wi * d * d;
         }
     }
@@ -1343,10 +1354,10 @@ void view_slice_ranges(const ctbb_geom *g, const std::vector<int64_t> &si_half,
     }
 }
 
-template <typename X, int MODE>
+template <typename X, int MODE, typename E = float>
 int launch_project(const ctbb_ssm *m, const X *x, const int64_t *d_si_half,
                    const int32_t *d_klo, const int32_t *d_khi, double *yhat,
-                   const float *y, const float *w, float *e, float *ew,
+                   const float *y, const float *w, E *e, E *ew,
                    double *partials,
                    hipStream_t st)
 {
@@ -1354,7 +1365,7 @@ int launch_project(const ctbb_ssm *m, const X *x, const int64_t *d_si_half,
     const ctbb_geom *g = &m->g;
     const int64_t ncg = (g->n_channels + kScatterCh - 1) / kScatterCh;
     const size_t lds = sizeof(double) * kScatterCh * g->n_rows;
-    hipLaunchKernelGGL((project_gather_kernel<X, MODE>), dim3((unsigned)project_grid(g)),
+    hipLaunchKernelGGL((project_gather_kernel<X, E, MODE>), dim3((unsigned)project_grid(g)),
                        dim3(kSweepBlock), lds, st, m->rm_vc, m->rm_ent, m->rm_val, x,
                        d_si_half, d_klo, d_khi, g->n_views_sm, g->n_channels, g->n_rows,
                        g->nx * g->ny, ncg, yhat, y, w, e, ew, partials);
@@ -1376,22 +1387,37 @@ int sum_partials(const double *d_part, int n, hipStream_t st, double *out)
     return CTBB_OK;
 }
 
-int recon_data_term(ctbb_recon *r, bool write_e, double *data)
+template <typename T>
+int recon_data_term_t(ctbb_recon *r, bool write_e, double *data)
 {
     const int g = (int)project_grid(&r->g);
-    CTBB_TRY((launch_project<float, 1>(r->m, r->x, r->si_half, r->k_lo, r->k_hi, nullptr,
-                                       r->y, r->w, write_e ? r->e : nullptr,
-                                       write_e ? r->ew : nullptr,
-                                       r->partials, r->stream)));
+    T *e = static_cast<T *>(r->e), *ew = static_cast<T *>(r->ew);
+    CTBB_TRY((launch_project<T, 1, T>(r->m, static_cast<const T *>(r->x), r->si_half,
+                                      r->k_lo, r->k_hi, nullptr, r->y, r->w,
+                                      write_e ? e : nullptr, write_e ? ew : nullptr,
+                                      r->partials, r->stream)));
     return sum_partials(r->partials, g, r->stream, data);
+}
+
+// Data term sum w (y - A x)^2 of the current volume (FP64 projection);
+// write_e: also store e = y - A x (and w e) as the running error sinogram.
+int recon_data_term(ctbb_recon *r, bool write_e, double *data)
+{
+    return r->fp64 ? recon_data_term_t<double>(r, write_e, data)
+                   : recon_data_term_t<float>(r, write_e, data);
 }
 
 int recon_penalty_term(ctbb_recon *r, double *pen)
 {
     const int g = red_grid(r->nvox);
-    hipLaunchKernelGGL(penalty_cost_kernel, dim3(g), dim3(kRedBlock), 0,
-                       r->stream, r->x, r->g.nx, r->nvox, r->opts.penalty,
-                       r->opts.delta, r->partials);
+    if (r->fp64)
+        hipLaunchKernelGGL(penalty_cost_kernel<double>, dim3(g), dim3(kRedBlock), 0,
+                           r->stream, static_cast<const double *>(r->x), r->g.nx, r->nvox,
+                           r->opts.penalty, r->opts.delta, r->partials);
+    else
+        hipLaunchKernelGGL(penalty_cost_kernel<float>, dim3(g), dim3(kRedBlock), 0,
+                           r->stream, static_cast<const float *>(r->x), r->g.nx, r->nvox,
+                           r->opts.penalty, r->opts.delta, r->partials);
     CTBB_HIP(hipGetLastError());
     return sum_partials(r->partials, g, r->stream, pen);
 }
@@ -1437,7 +1463,7 @@ extern "C" int ctbb_forward_project(const ctbb_ssm *m,
     if (e == hipSuccess) e = hipMemcpy(d_klo, klo.data(), g->readings * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_khi, khi.data(), g->readings * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        rc = launch_project<double, 0>(m, d_x, d_sih, d_klo, d_khi, d_y, nullptr, nullptr,
+        rc = launch_project<double, 0, float>(m, d_x, d_sih, d_klo, d_khi, d_y, nullptr, nullptr,
                                        nullptr, nullptr, nullptr, 0);
         if (rc == CTBB_OK)
             e = hipMemcpy(yhat, d_y, ds * 8, hipMemcpyDeviceToHost);
@@ -1518,8 +1544,7 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
 {
     if (!r)
         return CTBB_OK;
-    float *fl[] = {r->y, r->w, r->e, r->x};
-    for (float *p : fl)
+    for (void *p : {(void *)r->y, (void *)r->w, r->e, r->x})
         if (p) (void)hipFree(p);
     if (r->slice_off) (void)hipFree(r->slice_off);
     if (r->cols) (void)hipFree(r->cols);
@@ -1941,6 +1966,8 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     r->data_size = data_size_of(g);
     r->nvox = g->nx * g->ny * g->nz;
     r->h_slice_index.assign(slice_index, slice_index + g->nz);
+    r->fp64 = opts->state_fp64 != 0;
+    r->sb = r->fp64 ? 8 : 4;
     auto bail = [&](int rc) {
         ctbb_recon_free(r);
         return rc;
@@ -2013,17 +2040,17 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
 
     int rc = CTBB_OK;
     if ((rc = dev_alloc_n(&r->y, r->data_size, "sinogram")) ||
-        (rc = dev_alloc_n(&r->e, r->data_size, "error sinogram")) ||
-        (rc = dev_alloc_n(&r->x, r->nvox, "volume")) ||
+        (rc = dev_alloc(&r->e, r->data_size * r->sb, "error sinogram")) ||
+        (rc = dev_alloc(&r->x, r->nvox * r->sb, "volume")) ||
         (rc = dev_alloc_n(&r->slice_off, g->nz, "slice offsets")) ||
         (rc = dev_alloc_n(&r->cols, std::max<int64_t>(1, r->ncols_fov), "FOV columns")) ||
         (rc = dev_alloc_n(&r->partials, std::max<int64_t>(4096, project_grid(&r->g)),
                            "reduction partials")) ||
         (rc = dev_alloc_n(&r->batch_ptr, bptr.size(), "batch pointers")) ||
-        (rc = dev_alloc_n(&r->du, (max_batch + 8) * g->nz, "step updates")))
+        (rc = dev_alloc(&r->du, (max_batch + 8) * g->nz * r->sb, "step updates")))
         return bail(rc);
     if (weights && ((rc = dev_alloc_n(&r->w, r->data_size, "weights")) ||
-                    (rc = dev_alloc_n(&r->ew, r->data_size, "weighted error"))))
+                    (rc = dev_alloc(&r->ew, r->data_size * r->sb, "weighted error"))))
         return bail(rc);
 #define CTBB_HIP_R(call)                                                      \
     do {                                                                      \
@@ -2033,14 +2060,27 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
                                              hipGetErrorString(e_)));         \
     } while (0)
     CTBB_HIP_R(hipMemcpy(r->y, raw, r->data_size * 4, hipMemcpyHostToDevice));
-    CTBB_HIP_R(hipMemcpy(r->e, raw, r->data_size * 4, hipMemcpyHostToDevice));
+    // zero start: e = y (and w e)
+    if (r->fp64)
+        hipLaunchKernelGGL(widen_kernel<double>, dim3(red_grid(r->data_size)), dim3(kRedBlock),
+                           0, 0, r->y, r->data_size, static_cast<double *>(r->e));
+    else
+        hipLaunchKernelGGL(widen_kernel<float>, dim3(red_grid(r->data_size)), dim3(kRedBlock),
+                           0, 0, r->y, r->data_size, static_cast<float *>(r->e));
+    CTBB_HIP_R(hipGetLastError());
     if (weights) {
         CTBB_HIP_R(hipMemcpy(r->w, weights, r->data_size * 4, hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(weighted_error_kernel, dim3(red_grid(r->data_size)),
-                           dim3(kRedBlock), 0, 0, r->e, r->w, r->data_size, r->ew);
+        if (r->fp64)
+            hipLaunchKernelGGL(weighted_error_kernel<double>, dim3(red_grid(r->data_size)),
+                               dim3(kRedBlock), 0, 0, static_cast<const double *>(r->e),
+                               r->w, r->data_size, static_cast<double *>(r->ew));
+        else
+            hipLaunchKernelGGL(weighted_error_kernel<float>, dim3(red_grid(r->data_size)),
+                               dim3(kRedBlock), 0, 0, static_cast<const float *>(r->e),
+                               r->w, r->data_size, static_cast<float *>(r->ew));
         CTBB_HIP_R(hipGetLastError());
     }
-    CTBB_HIP_R(hipMemset(r->x, 0, r->nvox * 4));
+    CTBB_HIP_R(hipMemset(r->x, 0, r->nvox * r->sb));
     CTBB_HIP_R(hipMemcpy(r->slice_off, off.data(), g->nz * 8, hipMemcpyHostToDevice));
     r->slice_off_h = off;
     if (r->ncols_fov)
@@ -2091,8 +2131,8 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     {
         const int64_t ng_max = (g->nz + r->spacing - 1) / r->spacing;
         const int64_t np = std::max<int64_t>(1, ng_max * r->nib * max_batch);
-        if ((rc = dev_alloc_n(&r->part1, np, "partial sums")) ||
-            (rc = dev_alloc_n(&r->part2, np, "partial sums")) ||
+        if ((rc = dev_alloc(&r->part1, np * r->sb, "partial sums")) ||
+            (rc = dev_alloc(&r->part2, np * r->sb, "partial sums")) ||
             (rc = dev_alloc_n(&r->th2, r->nvox, "theta2")))
             return bail(rc);
         r->th2_valid = false;
@@ -2103,16 +2143,19 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
         const bool single = tile >= std::max(g->nx, g->ny);
         if (!opts->plain_curvature && !single) {
             const int64_t nd = (max_batch + 8) * g->nz;
-            if ((rc = dev_alloc_n(&r->ssum, r->data_size, "surrogate sample sums")) ||
-                (rc = dev_alloc_n(&r->du_m1, nd, "surrogate scatter weights")) ||
-                (rc = dev_alloc_n(&r->du_p1, nd, "surrogate scatter weights")))
+            if ((rc = dev_alloc(&r->ssum, r->data_size * r->sb, "surrogate sample sums")) ||
+                (rc = dev_alloc(&r->du_m1, nd * r->sb, "surrogate scatter weights")) ||
+                (rc = dev_alloc(&r->du_p1, nd * r->sb, "surrogate scatter weights")))
                 return bail(rc);
             std::vector<float> m1(nd, -1.0f), p1(nd, 1.0f);
-            hipError_t he2 = hipMemset(r->ssum, 0, r->data_size * 4);
+            std::vector<double> m1d(nd, -1.0), p1d(nd, 1.0);
+            const void *hm = r->fp64 ? (const void *)m1d.data() : (const void *)m1.data();
+            const void *hp = r->fp64 ? (const void *)p1d.data() : (const void *)p1.data();
+            hipError_t he2 = hipMemset(r->ssum, 0, r->data_size * r->sb);
             if (he2 == hipSuccess)
-                he2 = hipMemcpy(r->du_m1, m1.data(), nd * 4, hipMemcpyHostToDevice);
+                he2 = hipMemcpy(r->du_m1, hm, nd * r->sb, hipMemcpyHostToDevice);
             if (he2 == hipSuccess)
-                he2 = hipMemcpy(r->du_p1, p1.data(), nd * 4, hipMemcpyHostToDevice);
+                he2 = hipMemcpy(r->du_p1, hp, nd * r->sb, hipMemcpyHostToDevice);
             if (he2 != hipSuccess)
                 return bail(fail(CTBB_E_HIP, std::string("surrogate set-up: ") +
                                                  hipGetErrorString(he2)));
@@ -2136,12 +2179,69 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     return CTBB_OK;
 }
 
+// Host <-> device copies of state buffers of n values: `bytes_host` per
+// host value (4: float, 8: double), converted where the state type differs.
+static int state_upload(ctbb_recon *r, void *dst, const void *src, int64_t n, int host_b)
+{
+    if ((size_t)host_b == r->sb) {
+        CTBB_HIP(hipMemcpyAsync(dst, src, n * r->sb, hipMemcpyHostToDevice, r->stream));
+        CTBB_HIP(hipStreamSynchronize(r->stream));
+        return CTBB_OK;
+    }
+    if (host_b == 4) {   // float -> double state
+        std::vector<double> t(static_cast<const float *>(src),
+                              static_cast<const float *>(src) + n);
+        CTBB_HIP(hipMemcpyAsync(dst, t.data(), n * 8, hipMemcpyHostToDevice, r->stream));
+        CTBB_HIP(hipStreamSynchronize(r->stream));
+        return CTBB_OK;
+    }
+    std::vector<float> t(static_cast<const double *>(src),   // double -> float state
+                         static_cast<const double *>(src) + n);
+    CTBB_HIP(hipMemcpyAsync(dst, t.data(), n * 4, hipMemcpyHostToDevice, r->stream));
+    CTBB_HIP(hipStreamSynchronize(r->stream));
+    return CTBB_OK;
+}
+
+static int state_download(const ctbb_recon *r, void *dst, const void *src, int64_t n,
+                          int host_b)
+{
+    if ((size_t)host_b == r->sb) {
+        CTBB_HIP(hipMemcpyAsync(dst, src, n * r->sb, hipMemcpyDeviceToHost, r->stream));
+        CTBB_HIP(hipStreamSynchronize(r->stream));
+        return CTBB_OK;
+    }
+    if (host_b == 4) {   // double state -> float
+        std::vector<double> t(n);
+        CTBB_HIP(hipMemcpyAsync(t.data(), src, n * 8, hipMemcpyDeviceToHost, r->stream));
+        CTBB_HIP(hipStreamSynchronize(r->stream));
+        float *d = static_cast<float *>(dst);
+        for (int64_t i = 0; i < n; i++)
+            d[i] = (float)t[i];
+        return CTBB_OK;
+    }
+    std::vector<float> t(n);   // float state -> double
+    CTBB_HIP(hipMemcpyAsync(t.data(), src, n * 4, hipMemcpyDeviceToHost, r->stream));
+    CTBB_HIP(hipStreamSynchronize(r->stream));
+    double *d = static_cast<double *>(dst);
+    for (int64_t i = 0; i < n; i++)
+        d[i] = t[i];
+    return CTBB_OK;
+}
+
 extern "C" int ctbb_recon_set_volume(ctbb_recon *r, const float *volume)
 {
     if (!r || !volume)
         return fail(CTBB_E_ARG, "null argument");
-    CTBB_HIP(hipMemcpyAsync(r->x, volume, r->nvox * 4, hipMemcpyHostToDevice,
-                            r->stream));
+    CTBB_TRY(state_upload(r, r->x, volume, r->nvox, 4));
+    double d;
+    return recon_data_term(r, true, &d);
+}
+
+extern "C" int ctbb_recon_set_volume64(ctbb_recon *r, const double *volume)
+{
+    if (!r || !volume)
+        return fail(CTBB_E_ARG, "null argument");
+    CTBB_TRY(state_upload(r, r->x, volume, r->nvox, 8));
     double d;
     return recon_data_term(r, true, &d);
 }
@@ -2150,20 +2250,21 @@ extern "C" int ctbb_recon_get_volume(const ctbb_recon *r, float *volume)
 {
     if (!r || !volume)
         return fail(CTBB_E_ARG, "null argument");
-    CTBB_HIP(hipMemcpyAsync(volume, r->x, r->nvox * 4, hipMemcpyDeviceToHost,
-                            r->stream));
-    CTBB_HIP(hipStreamSynchronize(r->stream));
-    return CTBB_OK;
+    return state_download(r, volume, r->x, r->nvox, 4);
+}
+
+extern "C" int ctbb_recon_get_volume64(const ctbb_recon *r, double *volume)
+{
+    if (!r || !volume)
+        return fail(CTBB_E_ARG, "null argument");
+    return state_download(r, volume, r->x, r->nvox, 8);
 }
 
 extern "C" int ctbb_recon_get_error(const ctbb_recon *r, float *e)
 {
     if (!r || !e)
         return fail(CTBB_E_ARG, "null argument");
-    CTBB_HIP(hipMemcpyAsync(e, r->e, r->data_size * 4, hipMemcpyDeviceToHost,
-                            r->stream));
-    CTBB_HIP(hipStreamSynchronize(r->stream));
-    return CTBB_OK;
+    return state_download(r, e, r->e, r->data_size, 4);
 }
 
 extern "C" int ctbb_recon_resync(ctbb_recon *r)
@@ -2222,13 +2323,13 @@ extern "C" int ctbb_cost(const ctbb_ssm *m, const float *raw, const float *weigh
     if (e == hipSuccess) e = hipMemcpy(d_khi, khi.data(), g->readings * 4, hipMemcpyHostToDevice);
     double data = 0.0, pen = 0.0;
     if (e == hipSuccess) {
-        rc = launch_project<float, 1>(m, d_x, d_sih, d_klo, d_khi, nullptr, d_y, d_w, nullptr,
+        rc = launch_project<float, 1, float>(m, d_x, d_sih, d_klo, d_khi, nullptr, d_y, d_w, nullptr,
                                       nullptr, d_part, 0);
         if (rc == CTBB_OK)
             rc = sum_partials(d_part, (int)project_grid(g), 0, &data);
         if (rc == CTBB_OK) {
             const int gp = red_grid(nvox);
-            hipLaunchKernelGGL(penalty_cost_kernel, dim3(gp), dim3(kRedBlock), 0, 0, d_x,
+            hipLaunchKernelGGL(penalty_cost_kernel<float>, dim3(gp), dim3(kRedBlock), 0, 0, d_x,
                                g->nx, nvox, opts->penalty, opts->delta, d_part);
             e = hipGetLastError();
             if (e == hipSuccess)
@@ -2376,9 +2477,9 @@ extern "C" int ctbb_cost_noalloc(const ctbb_ssm *m, const float *raw,
                        (g->zffs || g->phiffs) ? 1 : 0, d_sih, d_klo, d_khi);
     CTBB_HIP(hipGetLastError());
     // every partial slot is written by its workgroup: no clearing needed
-    CTBB_TRY((launch_project<float, 1>(m, volume, d_sih, d_klo, d_khi, nullptr, raw, weights,
+    CTBB_TRY((launch_project<float, 1, float>(m, volume, d_sih, d_klo, d_khi, nullptr, raw, weights,
                                        nullptr, nullptr, d_pd, st)));
-    hipLaunchKernelGGL(penalty_cost_kernel, dim3((unsigned)npen), dim3(kRedBlock), 0, st,
+    hipLaunchKernelGGL(penalty_cost_kernel<float>, dim3((unsigned)npen), dim3(kRedBlock), 0, st,
                        volume, g->nx, nvox, opts->penalty, opts->delta, d_pp);
     hipLaunchKernelGGL(noalloc_finish_kernel, dim3(1), dim3(64), 0, st, d_pd, nproj, d_pp,
                        npen, opts->penalty, opts->lambda, cost);
@@ -2406,8 +2507,14 @@ extern "C" int ctbb_recon_cost_running(ctbb_recon *r, double *cost)
         return fail(CTBB_E_ARG, "null argument");
     double data = 0.0, pen = 0.0;
     const int g = red_grid(r->data_size);
-    hipLaunchKernelGGL(error_sq_kernel, dim3(g), dim3(kRedBlock), 0, r->stream, r->e, r->w,
-                       r->data_size, r->partials);
+    if (r->fp64)
+        hipLaunchKernelGGL(error_sq_kernel<double>, dim3(g), dim3(kRedBlock), 0, r->stream,
+                           static_cast<const double *>(r->e), r->w, r->data_size,
+                           r->partials);
+    else
+        hipLaunchKernelGGL(error_sq_kernel<float>, dim3(g), dim3(kRedBlock), 0, r->stream,
+                           static_cast<const float *>(r->e), r->w, r->data_size,
+                           r->partials);
     CTBB_HIP(hipGetLastError());
     CTBB_TRY(sum_partials(r->partials, g, r->stream, &data));
     CTBB_TRY(recon_penalty_term(r, &pen));
@@ -2418,10 +2525,9 @@ extern "C" int ctbb_recon_cost_running(ctbb_recon *r, double *cost)
     return CTBB_OK;
 }
 
-extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
+template <typename T>
+static int recon_iterate_t(ctbb_recon *r, int n, ctbb_icd_stats *stats)
 {
-    if (!r || n < 0)
-        return fail(CTBB_E_ARG, "bad argument");
     const ctbb_ssm *m = r->m;
     SweepArgs a{};
     a.col_ptr = m->col_ptr;
@@ -2473,7 +2579,7 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
     bt.nr = r->g.n_rows;
     bt.readings = r->g.readings;
     const int64_t ncg = (r->g.n_channels + kScatterCh - 1) / kScatterCh;
-    const size_t scatter_lds = sizeof(float) * kScatterViews * kScatterCh * r->g.n_rows;
+    const size_t scatter_lds = sizeof(T) * kScatterViews * kScatterCh * r->g.n_rows;
 
     hipEvent_t e0 = r->ev_sweep[0], e1 = r->ev_sweep[1];
     double sweep_ms = 0.0, resync_ms = 0.0;
@@ -2489,14 +2595,14 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
     for (int it = 0; it < n; it++) {
         // the first sweep also computes and caches theta2
         const bool first = !r->th2_valid;
-        auto gather = fp16 ? (first ? icd_gather_kernel<true, true>
-                                    : icd_gather_kernel<true, false>)
-                           : (first ? icd_gather_kernel<false, true>
-                                    : icd_gather_kernel<false, false>);
-        auto solve = fp16 ? (first ? icd_solve_kernel<true, true>
-                                   : icd_solve_kernel<true, false>)
-                          : (first ? icd_solve_kernel<false, true>
-                                   : icd_solve_kernel<false, false>);
+        auto gather = fp16 ? (first ? icd_gather_kernel<T, true, true>
+                                    : icd_gather_kernel<T, true, false>)
+                           : (first ? icd_gather_kernel<T, false, true>
+                                    : icd_gather_kernel<T, false, false>);
+        auto solve = fp16 ? (first ? icd_solve_kernel<T, true, true>
+                                   : icd_solve_kernel<T, true, false>)
+                          : (first ? icd_solve_kernel<T, false, true>
+                                   : icd_solve_kernel<T, false, false>);
         CTBB_HIP(hipEventRecord(e0, r->stream));
         int64_t step = 0, used = 0;
         for (int64_t b = 0; b < r->nbatch; b++) {
@@ -2542,7 +2648,7 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                 as.w = nullptr;
                 if (first && r->ssum && scatter_grid > 0) {
                     as.du = r->du_m1;
-                    hipLaunchKernelGGL(icd_scatter_kernel, dim3((unsigned)scatter_grid),
+                    hipLaunchKernelGGL(icd_scatter_kernel<T>, dim3((unsigned)scatter_grid),
                                        dim3(kSweepBlock), scatter_lds, r->stream, as, sa, bt,
                                        b, so);
                 }
@@ -2559,7 +2665,7 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                 if (samp)
                     CTBB_HIP(hipEventRecord(ev[4 * used + 2], r->stream));
                 if (scatter_grid > 0)
-                    hipLaunchKernelGGL(icd_scatter_kernel, dim3((unsigned)scatter_grid),
+                    hipLaunchKernelGGL(icd_scatter_kernel<T>, dim3((unsigned)scatter_grid),
                                        dim3(kSweepBlock), scatter_lds, r->stream, a, sa, bt, b,
                                        so);
                 if (samp) {
@@ -2569,7 +2675,7 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                 // ... and S -= a in the same order: exactly zero again
                 if (first && r->ssum && scatter_grid > 0) {
                     as.du = r->du_p1;
-                    hipLaunchKernelGGL(icd_scatter_kernel, dim3((unsigned)scatter_grid),
+                    hipLaunchKernelGGL(icd_scatter_kernel<T>, dim3((unsigned)scatter_grid),
                                        dim3(kSweepBlock), scatter_lds, r->stream, as, sa, bt,
                                        b, so);
                 }
@@ -2620,4 +2726,11 @@ extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
         stats->group_spacing = r->spacing;
     }
     return CTBB_OK;
+}
+
+extern "C" int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats)
+{
+    if (!r || n < 0)
+        return fail(CTBB_E_ARG, "bad argument");
+    return r->fp64 ? recon_iterate_t<double>(r, n, stats) : recon_iterate_t<float>(r, n, stats);
 }

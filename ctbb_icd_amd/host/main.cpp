@@ -58,6 +58,7 @@ static void empty_parameter_file()
            "resync_every:       # FP64 error-sinogram resync period (iterations)\n"
            "tile:               # batch tile of the parallel sweep (default 4)\n"
            "plain_curvature:    # 1: plain Jacobi curvature in batches (no surrogate)\n"
+           "state_fp64:         # 1: FP64 image and error sinogram in the sweep\n"
         << std::endl;
     exit(0);
 }

@@ -214,6 +214,7 @@ void icd_iteration(const struct recon_params *rp, struct ct_data *data)
     o.resync_every = (int32_t)rp->resync_every;
     o.tile = (int32_t)rp->tile;
     o.plain_curvature = (int32_t)rp->plain_curvature;
+    o.state_fp64 = (int32_t)rp->state_fp64;
     ctbb_recon *r = nullptr;
     if (ctbb_recon_create(data->ssm, data->raw, data->weights, si.data(), &o, &r) !=
         CTBB_OK)

@@ -252,6 +252,7 @@ int parse_config_checked(const std::string &config_file, recon_params *rp)
     Q(group_spacing, size_t);
     Q(tile, size_t);
     Q(plain_curvature, size_t);
+    Q(state_fp64, size_t);
 #undef Q
     return CTBB_OK;
 }
@@ -286,6 +287,7 @@ static void set_extension_defaults(recon_params *rp)
     rp->group_spacing = 0;
     rp->tile = 0;
     rp->plain_curvature = 0;
+    rp->state_fp64 = 0;
 }
 
 int configure_recon_params_checked(const std::string &filename, recon_params *out)
@@ -597,6 +599,7 @@ static void fill_run(const recon_params &rp, ctbb_run_params *p)
     p->ssm_fp16 = (int32_t)rp.ssm_fp16;
     p->neighbours = (int32_t)rp.neighbours;
     p->resync_every = (int32_t)rp.resync_every;
+    p->state_fp64 = (int32_t)rp.state_fp64;
     p->group_spacing = (int64_t)rp.group_spacing;
 }
 

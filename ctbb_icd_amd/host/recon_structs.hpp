@@ -117,6 +117,7 @@ struct recon_params {
     size_t group_spacing;          // 0: derive from the SSM footprints
     size_t tile;                   // batch tile of the sweep (0: default)
     size_t plain_curvature;        // 1: no batch surrogate (ctbb_icd_opts)
+    size_t state_fp64;             // 1: FP64 sweep state (ctbb_icd_opts)
     std::string prm_path;          // the parameter file itself
 };
 

@@ -29,7 +29,7 @@ typedef struct ctbb_run_params {
     int32_t ssm_fp16;
     int32_t neighbours;
     int32_t resync_every;
-    int32_t reserved;
+    int32_t state_fp64;         /* FP64 sweep state (ctbb_icd_opts)         */
     int64_t group_spacing;
 } ctbb_run_params;
 

@@ -199,6 +199,13 @@ typedef struct ctbb_icd_opts {
                                  1: sum_r w_r a_r^2 in every batch (plain
                                  Jacobi across the step; can overshoot when
                                  the weighted data term dominates). */
+    int32_t state_fp64;       /* 0 (default): FP32 sweep state (error
+                                 sinogram, volume, updates, partial sums);
+                                 1: FP64 state -- x and e = y - A x kept in
+                                 FP64 like the reference's image and
+                                 sinogram estimate (src/icd_iteration.cpp:
+                                 47-48), for converged reference runs
+                                 (twice the state bytes per sweep). */
 } ctbb_icd_opts;
 
 typedef struct ctbb_icd_stats {
@@ -234,6 +241,10 @@ int ctbb_recon_free(ctbb_recon *r);
  * when wfbp_initialize is set, src/icd_iteration.cpp:63; this always does). */
 int ctbb_recon_set_volume(ctbb_recon *r, const float *volume);
 int ctbb_recon_get_volume(const ctbb_recon *r, float *volume);
+/* FP64 forms of set/get volume (exact for the FP64 state; converted to and
+ * from FP32 for the FP32 state). */
+int ctbb_recon_set_volume64(ctbb_recon *r, const double *volume);
+int ctbb_recon_get_volume64(const ctbb_recon *r, double *volume);
 /* Run n ICD iterations (src/icd_iteration.cpp:149-274). */
 int ctbb_recon_iterate(ctbb_recon *r, int n, ctbb_icd_stats *stats);
 /* Recompute e = y - A x in FP64 and store it as FP32. */
@@ -253,7 +264,7 @@ int ctbb_recon_cost(ctbb_recon *r, double *cost);
  * rounded FP64 projection residual); for cheap comparisons of candidate
  * volumes and for monitoring.  No reference counterpart. */
 int ctbb_recon_cost_running(ctbb_recon *r, double *cost);
-/* Copy the running FP32 error sinogram to the host. */
+/* Copy the running error sinogram to the host (rounded to FP32). */
 int ctbb_recon_get_error(const ctbb_recon *r, float *e);
 
 /* FP64 cost of a volume (rotated frame, float32 [nx*ny*nz]) for raw data,
