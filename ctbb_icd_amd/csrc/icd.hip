@@ -325,7 +325,10 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
     constexpr int kAcc = 64 * kViewBlock;
     constexpr int M = THETA2 ? kGatherMembersFirst : kGatherMembers;
     __shared__ T accm[THETA2 ? 2 * M : M][kWaves * kAcc];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // wave index pinned to a scalar register: the per-wave chunk bounds and
+    // bases below are then scalar too
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t nwg = g.nib * g.nsc;
     const int64_t per_xcd = (nwg + kXcd - 1) / kXcd;
     const int64_t L = (int64_t)(blockIdx.x % kXcd) * per_xcd + blockIdx.x / kXcd;
@@ -371,16 +374,27 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
 #pragma unroll
             for (int q = 0; q < kViewBlock; q++)
                 accm[m][wave * kAcc + q * 64 + lane] = T(0);
-        for (int64_t p0 = lo; p0 < hi; p0 += 64 * kGatherUnroll) {
+        // the chunk's entries from scalar bases with 32-bit lane offsets,
+        // clamped into the chunk (a chunk is non-empty inside the loop)
+        const int32_t *idx_c = g.idx + lo;
+        const float *val_c = FP16 ? nullptr : g.val + lo;
+        const __half *val16_c = FP16 ? g.val16 + lo : nullptr;
+        const uint8_t *key_c = g.key + lo;
+        const uint32_t n = (uint32_t)(hi - lo);
+        for (uint32_t p0 = 0; p0 < n; p0 += 64 * kGatherUnroll) {
             int32_t iv[kGatherUnroll], kv[kGatherUnroll];
             float av[kGatherUnroll];
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; u++) {
-                const int64_t p = p0 + u * 64 + lane;
-                const bool ok = p < hi;
-                iv[u] = ok ? g.idx[p] : 0;
-                av[u] = ok ? gb_value<FP16>(g, p) : 0.0f;
-                kv[u] = ok ? (int32_t)g.key[p] : -1;
+                const uint32_t p = p0 + u * 64 + lane;
+                const bool ok = p < n;
+                const uint32_t pc = ok ? p : 0u;
+                const int32_t i0 = idx_c[pc];
+                const float a0 = FP16 ? __half2float(val16_c[pc]) : val_c[pc];
+                const int32_t k0 = (int32_t)key_c[pc];
+                iv[u] = ok ? i0 : 0;
+                av[u] = ok ? a0 : 0.0f;
+                kv[u] = ok ? k0 : -1;
             }
             T ev[M][kGatherUnroll], sv[M][kGatherUnroll];
             float mv[M][kGatherUnroll], wv[M][kGatherUnroll];
@@ -391,19 +405,20 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                     // |ray index| + |slice offset| < 2^31 (checked at set-up)
                     const int32_t r = iv[u] + off[m];
                     const bool in = has[m] && kv[u] >= 0 && (uint32_t)r < ds;
+                    const uint32_t rc = in ? (uint32_t)r : 0u;
                     mv[m][u] = in ? av[u] : 0.0f;
-                    ev[m][u] = ew[in ? r : 0];
-                    wv[m][u] = (THETA2 && wt) ? wt[in ? r : 0] : 1.0f;
+                    ev[m][u] = ew[rc];
+                    wv[m][u] = (THETA2 && wt) ? wt[rc] : 1.0f;
                     // curvature factor: the step's summed values on this
                     // sample (surrogate), or a itself (a^2 squared in FP32,
                     // as src/icd_iteration.cpp:208)
                     if (THETA2)
-                        sv[m][u] = ssum ? (T)mv[m][u] * ssum[in ? r : 0]
+                        sv[m][u] = ssum ? (T)mv[m][u] * ssum[rc]
                                         : (T)(mv[m][u] * av[u]);
                 }
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; u++) {
-                if (p0 + u * 64 >= hi)
+                if (p0 + u * 64 >= n)
                     break;
                 const uint64_t head = run_heads(kv[u], lane);
                 T t1[M], t2[M];
@@ -601,7 +616,8 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
     T *acc = reinterpret_cast<T *>(acc_raw);
     const T *du = static_cast<const T *>(a.du);
     __shared__ int any_work;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t nwg = so.P * so.rows * so.ncg;
     const int64_t per_xcd = (nwg + kXcd - 1) / kXcd;
     const int64_t L = (int64_t)(blockIdx.x % kXcd) * per_xcd + blockIdx.x / kXcd;
@@ -648,29 +664,32 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
         // (item, 64-entry chunk) pairs in item order, kItemUnroll at a time:
         // the loads of several chunks -- of one long item or of several
         // short ones -- are in flight before any is reduced; chunks are
-        // applied in stream order (deterministic)
+        // applied in stream order (deterministic).  The schedule is
+        // wave-uniform and kept in scalar registers (readlane of the lane
+        // that holds the item); loads use uniform base pointers with 32-bit
+        // lane offsets, clamped into the chunk instead of branched around.
         int cur = -1;
         int32_t clo = 0, chi = 0;
         while (true) {
             int it[kItemUnroll];
-            int32_t lo_u[kItemUnroll], hi_u[kItemUnroll];
+            int32_t lo_u[kItemUnroll], n_u[kItemUnroll];
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
                 if (clo >= chi && todo) {
                     cur = __ffsll((unsigned long long)todo) - 1;
                     todo &= todo - 1;
-                    clo = __shfl(lo, cur, 64);
-                    chi = __shfl(hi, cur, 64);
+                    clo = __builtin_amdgcn_readlane(lo, cur);
+                    chi = __builtin_amdgcn_readlane(hi, cur);
                 }
                 if (clo < chi) {
                     it[u] = cur;
                     lo_u[u] = clo;
-                    hi_u[u] = min(clo + 64, chi);
+                    n_u[u] = min(64, chi - clo);
                     clo += 64;
                 } else {
                     it[u] = -1;
                     lo_u[u] = 0;
-                    hi_u[u] = 0;
+                    n_u[u] = 0;
                 }
             }
             if (it[0] < 0)
@@ -680,17 +699,20 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
             bool ok[kItemUnroll];
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
-                const int32_t p = lo_u[u] + lane;
-                ok[u] = p < hi_u[u];
-                ev[u] = ok[u] ? ent[p] : 0u;
-                av[u] = ok[u] ? val[p] : 0.0f;
+                ok[u] = lane < n_u[u];
+                const uint32_t *ent_u = ent + lo_u[u];
+                const float *val_u = val + lo_u[u];
+                const uint32_t pc = ok[u] ? (uint32_t)lane : 0u;
+                ev[u] = it[u] >= 0 ? ent_u[pc] : 0u;
+                av[u] = it[u] >= 0 ? val_u[pc] : 0.0f;
             }
             T dv[kItemUnroll];
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
                 const int32_t vm = (int32_t)vb + (it[u] < 0 ? 0 : (it[u] % kItemLanes));
-                dv[u] = ok[u] ? du[vm * (int32_t)sa.nc + (int32_t)bt_slot(ev[u])] * (T)av[u]
-                              : T(0);
+                const T *du_row = du + (uint32_t)(vm * (int32_t)sa.nc);
+                const T d = du_row[bt_slot(ev[u])] * (T)av[u];
+                dv[u] = ok[u] ? d : T(0);
             }
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {

@@ -504,6 +504,232 @@ double orc_edge_preserving(const ctbb_geom *g, int64_t q, double alpha,
 }
 
 /* ------------------------------------------------------------------------ */
+/* 26-neighbour penalty (no reference counterpart: the reference stencil is  */
+/* 8 in-plane neighbours, "limited to 2D", src/icd_iteration.cpp:140).       */
+/* The slices are rotated: voxel p of slice k sits at R(theta_k) p (theta_k  */
+/* the tube angle of its central view), so the point of slice k+1 above it   */
+/* is R(theta_k - theta_{k+1}) p.  That rotation is taken as three shears    */
+/* with integer-rounded shifts (Paeth): an exact bijection of the grid with  */
+/* an exact inverse.  Pairs: in-plane the 8 neighbours without wrap, and     */
+/* (p in slice k, pi_k(p) + d in slice k+1) for the 9 offsets d of a 3x3     */
+/* block; weights 1/|offset| in voxel units (1, 1/sqrt2, 1/sqrt3).            */
+static double orc_rnd(double v) { return floor(v + 0.5); }
+
+/* shear parameters of pi_k (slices k -> k+1): a = -tan(phi/2), b = sin(phi),
+ * phi = theta_k - theta_{k+1} = -(c_{k+1} - c_k) * tube_angle_increment */
+static void orc_shears(const ctbb_geom *g, const int64_t *slice_index,
+                       double *sh)
+{
+    for (int64_t k = 0; k + 1 < g->nz; k++) {
+        double phi = -(double)(slice_index[k + 1] - slice_index[k])
+                     * g->tube_angle_increment;
+        sh[2 * k] = -tan(phi / 2.0);
+        sh[2 * k + 1] = sin(phi);
+    }
+}
+
+static void orc_rot_fwd(const ctbb_geom *g, const double *sh, int64_t k,
+                        int64_t i, int64_t j, int64_t *io, int64_t *jo)
+{
+    double a = sh[2 * k], b = sh[2 * k + 1];
+    double x = (double)i - g->center_voxel_x, y = (double)j - g->center_voxel_y;
+    double x1 = x + orc_rnd(a * y);
+    double y2 = y + orc_rnd(b * x1);
+    double x3 = x1 + orc_rnd(a * y2);
+    *io = (int64_t)(x3 + g->center_voxel_x);
+    *jo = (int64_t)(y2 + g->center_voxel_y);
+}
+
+static void orc_rot_inv(const ctbb_geom *g, const double *sh, int64_t k,
+                        int64_t i, int64_t j, int64_t *io, int64_t *jo)
+{
+    double a = sh[2 * k], b = sh[2 * k + 1];
+    double x3 = (double)i - g->center_voxel_x, y2 = (double)j - g->center_voxel_y;
+    double x1 = x3 - orc_rnd(a * y2);
+    double y1 = y2 - orc_rnd(b * x1);
+    double x = x1 - orc_rnd(a * y1);
+    *io = (int64_t)(x + g->center_voxel_x);
+    *jo = (int64_t)(y1 + g->center_voxel_y);
+}
+
+void orc_rot26(const ctbb_geom *g, const int64_t *slice_index, int64_t k,
+               int inverse, int64_t i, int64_t j, int64_t *io, int64_t *jo)
+{
+    double *sh = (double *)malloc(sizeof(double) * 2 * (g->nz > 1 ? g->nz - 1 : 1));
+    orc_shears(g, slice_index, sh);
+    if (inverse)
+        orc_rot_inv(g, sh, k, i, j, io, jo);
+    else
+        orc_rot_fwd(g, sh, k, i, j, io, jo);
+    free(sh);
+}
+
+/* The 26-neighbourhood of voxel q: up to 26 indices and weights. */
+static int orc_nbrs26(const ctbb_geom *g, const double *sh, int64_t q,
+                      int64_t *nb, double *w)
+{
+    int64_t nx = g->nx, ny = g->ny, nxy = nx * ny;
+    int64_t k = q / nxy, c = q - k * nxy, j = c / nx, i = c - j * nx;
+    int n = 0;
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+            if (!dx && !dy)
+                continue;
+            int64_t ii = i + dx, jj = j + dy;
+            if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+                nb[n] = ii + nx * jj + nxy * k;
+                w[n++] = (dx && dy) ? 1.0 / ORC_SQRT2 : 1.0;
+            }
+        }
+    if (k + 1 < g->nz) {
+        int64_t iu, ju;
+        orc_rot_fwd(g, sh, k, i, j, &iu, &ju);
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                int64_t ii = iu + dx, jj = ju + dy;
+                if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+                    nb[n] = ii + nx * jj + nxy * (k + 1);
+                    w[n++] = 1.0 / sqrt((double)(dx * dx + dy * dy + 1));
+                }
+            }
+    }
+    if (k >= 1) {
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                int64_t ii, jj;
+                orc_rot_inv(g, sh, k - 1, i - dx, j - dy, &ii, &jj);
+                if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+                    nb[n] = ii + nx * jj + nxy * (k - 1);
+                    w[n++] = 1.0 / sqrt((double)(dx * dx + dy * dy + 1));
+                }
+            }
+    }
+    return n;
+}
+
+/* Quadratic step over the neighbours actually present (the exact
+ * coordinate minimiser; the 8-neighbour reference form divides by the full
+ * weight sum, src/penalties.cpp:76-84). */
+static double orc_quadratic26(const ctbb_geom *g, const double *sh, int64_t q,
+                              double alpha, double beta, double lambda,
+                              const double *x)
+{
+    int64_t nb[26];
+    double w[26];
+    int n = orc_nbrs26(g, sh, q, nb, w);
+    double sum = 0.0, W = 0.0;
+    for (int t = 0; t < n; t++) {
+        sum += w[t] * (x[nb[t]] - x[q]);
+        W += w[t];
+    }
+    return (beta + lambda * sum) / (alpha + lambda * W);
+}
+
+/* Edge-preserving step: the bisection of src/penalties.cpp:88-139 over the
+ * 26-neighbourhood. */
+static double orc_edge_preserving26(const ctbb_geom *g, const double *sh,
+                                    int64_t q, double alpha, double beta,
+                                    double lambda, double delta, const double *x)
+{
+    if (alpha == 0.0)
+        return 0.0;
+    int64_t nb[26];
+    double w[26], diff[26];
+    int n = orc_nbrs26(g, sh, q, nb, w);
+    double a = beta / alpha, b = beta / alpha;
+    for (int t = 0; t < n; t++) {
+        diff[t] = x[nb[t]] - x[q];
+        if (diff[t] < a) a = diff[t];
+        if (diff[t] > b) b = diff[t];
+    }
+    const double tol = 0.00000001;
+    double c = (a + b) / 2.0;
+    for (int it = 0; it < 1000000; it++) {
+        c = (a + b) / 2.0;
+        double fc = 0.0, fa = 0.0;
+        for (int t = 0; t < n; t++) {
+            double u = c - diff[t], v = a - diff[t];
+            fc += (u >= 0 ? 1.0 : -1.0) * w[t] * (1.0 - 1.0 / (1.0 + fabs(u / delta)));
+            fa += (v >= 0 ? 1.0 : -1.0) * w[t] * (1.0 - 1.0 / (1.0 + fabs(v / delta)));
+        }
+        fc = 2 * alpha * c - 2 * beta + lambda * fc;
+        fa = 2 * alpha * a - 2 * beta + lambda * fa;
+        if ((fc == 0) || (b - a) / 2 < tol)
+            return c;
+        if (orc_sign(fc) == orc_sign(fa))
+            a = c;
+        else
+            b = c;
+    }
+    return c;
+}
+
+double orc_penalty_step(const ctbb_geom *g, const int64_t *slice_index,
+                        int neighbours, int penalty, int64_t q, double alpha,
+                        double beta, double lambda, double delta, const double *x)
+{
+    if (neighbours != 26)
+        return penalty == CTBB_PENALTY_QUADRATIC
+                   ? orc_quadratic(g, q, alpha, beta, lambda, x)
+                   : orc_edge_preserving(g, q, alpha, beta, lambda, delta, x);
+    double *sh = (double *)malloc(sizeof(double) * 2 * (g->nz > 1 ? g->nz - 1 : 1));
+    orc_shears(g, slice_index, sh);
+    double du = penalty == CTBB_PENALTY_QUADRATIC
+                    ? orc_quadratic26(g, sh, q, alpha, beta, lambda, x)
+                    : orc_edge_preserving26(g, sh, q, alpha, beta, lambda, delta, x);
+    free(sh);
+    return du;
+}
+
+/* Penalty sum over the 26-neighbour pairs, each pair once: in-plane
+ * (i+1, j), (i-1, j+1), (i, j+1), (i+1, j+1) inside the slice; across
+ * slices (p, pi_k(p) + d). */
+static double orc_penalty26(const ctbb_geom *g, const double *sh, int penalty,
+                            double delta, const double *x)
+{
+    int64_t nx = g->nx, ny = g->ny, nxy = nx * ny;
+    double pen = 0.0;
+    for (int64_t k = 0; k < g->nz; k++)
+        for (int64_t j = 0; j < ny; j++)
+            for (int64_t i = 0; i < nx; i++) {
+                int64_t q = i + nx * j + nxy * k;
+                int64_t pr[13];
+                double pw[13];
+                int n = 0;
+                const int ox[4] = {1, -1, 0, 1}, oy[4] = {0, 1, 1, 1};
+                for (int t = 0; t < 4; t++) {
+                    int64_t ii = i + ox[t], jj = j + oy[t];
+                    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+                        pr[n] = ii + nx * jj + nxy * k;
+                        pw[n++] = (ox[t] && oy[t]) ? 1.0 / ORC_SQRT2 : 1.0;
+                    }
+                }
+                if (k + 1 < g->nz) {
+                    int64_t iu, ju;
+                    orc_rot_fwd(g, sh, k, i, j, &iu, &ju);
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dx = -1; dx <= 1; dx++) {
+                            int64_t ii = iu + dx, jj = ju + dy;
+                            if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+                                pr[n] = ii + nx * jj + nxy * (k + 1);
+                                pw[n++] = 1.0 / sqrt((double)(dx * dx + dy * dy + 1));
+                            }
+                        }
+                }
+                for (int t = 0; t < n; t++) {
+                    double diff = x[q] - x[pr[t]];
+                    if (penalty == CTBB_PENALTY_QUADRATIC) {
+                        pen += pw[t] * diff * diff;
+                    } else {
+                        double a = fabs(diff);
+                        pen += pw[t] * (a - delta * log1p(a / delta));
+                    }
+                }
+            }
+    return pen;
+}
+
+/* ------------------------------------------------------------------------ */
 /* ICD: src/icd_iteration.cpp:149-274 (plus optional weights and the        */
 /* batched z-group order).                                                   */
 /* theta sums and penalty step of voxel (c, k) against the current x, yhat.
@@ -517,7 +743,8 @@ static double orc_voxel_step(const ctbb_geom *g, const int64_t *col_ptr,
                              const float *raw, const float *weights, int64_t c,
                              int64_t k, int64_t off, int penalty,
                              double lambda, double delta, const double *x,
-                             const double *yhat, const double *S)
+                             const double *yhat, const double *S,
+                             const double *sh26)
 {
     int64_t data_size = g->readings * g->n_channels * g->n_rows;
     int64_t q = c + g->nx * g->ny * k;
@@ -538,7 +765,11 @@ static double orc_voxel_step(const ctbb_geom *g, const int64_t *col_ptr,
         }
     }
     double du;
-    if (penalty == CTBB_PENALTY_QUADRATIC)
+    if (sh26)
+        du = penalty == CTBB_PENALTY_QUADRATIC
+                 ? orc_quadratic26(g, sh26, q, alpha, beta, lambda, x)
+                 : orc_edge_preserving26(g, sh26, q, alpha, beta, lambda, delta, x);
+    else if (penalty == CTBB_PENALTY_QUADRATIC)
         du = orc_quadratic(g, q, alpha, beta, lambda, x);
     else
         du = orc_edge_preserving(g, q, alpha, beta, lambda, delta, x);
@@ -568,7 +799,7 @@ static void orc_apply(const ctbb_geom *g, const int64_t *col_ptr,
 static double orc_cost_state(const ctbb_geom *g, const float *raw,
                              const float *weights, int penalty, double lambda,
                              double delta, const double *x, const double *yhat,
-                             int par)
+                             int par, const double *sh26)
 {
     int64_t data_size = g->readings * g->n_channels * g->n_rows;
     double data = 0.0;
@@ -582,6 +813,8 @@ static double orc_cost_state(const ctbb_geom *g, const float *raw,
     int64_t dd[4] = {1, nx - 1, nx, nx + 1};
     double wd[4] = {1.0, 1.0 / ORC_SQRT2, 1.0, 1.0 / ORC_SQRT2};
     double pen = 0.0;
+    if (sh26)
+        n = 0;   /* the 26-neighbour pairs instead of the flat stencil */
     for (int64_t q = 0; q < n; q++) {
         for (int t = 0; t < 4; t++) {
             if (q + dd[t] >= n)
@@ -595,6 +828,8 @@ static double orc_cost_state(const ctbb_geom *g, const float *raw,
             }
         }
     }
+    if (sh26)
+        pen = orc_penalty26(g, sh26, penalty, delta, x);
     if (penalty == CTBB_PENALTY_QUADRATIC)
         return 0.5 * data + 0.5 * lambda * pen;
     return data + lambda * pen;
@@ -631,7 +866,25 @@ int64_t orc_icd_trace(const ctbb_geom *g, const int64_t *col_ptr,
                       double *yhat, double *step_cost, int64_t max_costs,
                       int64_t *n_steps, int64_t *shared)
 {
+    return orc_icd_nb(g, col_ptr, idx, val, raw, weights, slice_index, penalty,
+                      lambda, delta, 8, spacing, tile, surrogate, n_iter, x, yhat,
+                      step_cost, max_costs, n_steps, shared);
+}
+
+int64_t orc_icd_nb(const ctbb_geom *g, const int64_t *col_ptr,
+                   const int32_t *idx, const float *val, const float *raw,
+                   const float *weights, const int64_t *slice_index,
+                   int penalty, double lambda, double delta, int neighbours,
+                   int64_t spacing, int64_t tile, int surrogate, int n_iter,
+                   double *x, double *yhat, double *step_cost, int64_t max_costs,
+                   int64_t *n_steps, int64_t *shared)
+{
     int64_t updates = 0;
+    double *sh26 = NULL;
+    if (neighbours == 26) {
+        sh26 = (double *)malloc(sizeof(double) * 2 * (g->nz > 1 ? g->nz - 1 : 1));
+        orc_shears(g, slice_index, sh26);
+    }
     int64_t data_size = g->readings * g->n_channels * g->n_rows;
     double fov_limit = (g->acquisition_fov / 2.0) * (g->acquisition_fov / 2.0);
     int64_t *offs = (int64_t *)malloc(sizeof(int64_t) * g->nz);
@@ -656,7 +909,7 @@ int64_t orc_icd_trace(const ctbb_geom *g, const int64_t *col_ptr,
                         double du = orc_voxel_step(g, col_ptr, idx, val, raw,
                                                    weights, c, k, offs[k],
                                                    penalty, lambda, delta, x,
-                                                   yhat, NULL);
+                                                   yhat, NULL, sh26);
                         orc_apply(g, col_ptr, idx, val, c, k, offs[k], du, x,
                                   yhat);
                         updates++;
@@ -664,6 +917,7 @@ int64_t orc_icd_trace(const ctbb_geom *g, const int64_t *col_ptr,
                 }
             }
         free(offs);
+        free(sh26);
         return updates;
     }
     /* batched z-group order: batches of columns at one local position of
@@ -693,7 +947,8 @@ int64_t orc_icd_trace(const ctbb_geom *g, const int64_t *col_ptr,
     }
     int64_t nst = 0;
     if (step_cost && max_costs > 0)
-        step_cost[0] = orc_cost_state(g, raw, weights, penalty, lambda, delta, x, yhat, 1);
+        step_cost[0] = orc_cost_state(g, raw, weights, penalty, lambda, delta, x, yhat, 1,
+                                      sh26);
     for (int n = 0; n < n_iter; n++) {
         for (int64_t ly = 0; ly < tile; ly++)
             for (int64_t lx = 0; lx < tile; lx++) {
@@ -739,7 +994,7 @@ int64_t orc_icd_trace(const ctbb_geom *g, const int64_t *col_ptr,
                             dus[t++] = orc_voxel_step(g, col_ptr, idx, val, raw,
                                                       weights, bcols[b], k,
                                                       offs[k], penalty, lambda,
-                                                      delta, x, yhat, S);
+                                                      delta, x, yhat, S, sh26);
                     t = 0;
                     for (int64_t b = 0; b < nb; b++)
                         for (int64_t k = jj; k < g->nz; k += spacing) {
@@ -762,7 +1017,7 @@ int64_t orc_icd_trace(const ctbb_geom *g, const int64_t *col_ptr,
                     nst++;
                     if (step_cost && nst < max_costs)
                         step_cost[nst] = orc_cost_state(g, raw, weights, penalty,
-                                                        lambda, delta, x, yhat, 1);
+                                                        lambda, delta, x, yhat, 1, sh26);
                 }
             }
     }
@@ -774,6 +1029,7 @@ int64_t orc_icd_trace(const ctbb_geom *g, const int64_t *col_ptr,
     free(mk_k);
     free(mk_c);
     free(offs);
+    free(sh26);
     return updates;
 }
 
@@ -788,8 +1044,28 @@ double orc_cost(const ctbb_geom *g, const int64_t *col_ptr,
     int64_t data_size = g->readings * g->n_channels * g->n_rows;
     double *yhat = (double *)malloc(sizeof(double) * data_size);
     orc_forward_project(g, col_ptr, idx, val, slice_index, x, yhat);
-    double c = orc_cost_state(g, raw, weights, penalty, lambda, delta, x, yhat, 0);
+    double c = orc_cost_state(g, raw, weights, penalty, lambda, delta, x, yhat, 0, NULL);
     free(yhat);
+    return c;
+}
+
+double orc_cost_nb(const ctbb_geom *g, const int64_t *col_ptr,
+                   const int32_t *idx, const float *val, const float *raw,
+                   const float *weights, const int64_t *slice_index,
+                   int penalty, double lambda, double delta, int neighbours,
+                   const double *x)
+{
+    if (neighbours != 26)
+        return orc_cost(g, col_ptr, idx, val, raw, weights, slice_index, penalty,
+                        lambda, delta, x);
+    int64_t data_size = g->readings * g->n_channels * g->n_rows;
+    double *yhat = (double *)malloc(sizeof(double) * data_size);
+    double *sh = (double *)malloc(sizeof(double) * 2 * (g->nz > 1 ? g->nz - 1 : 1));
+    orc_shears(g, slice_index, sh);
+    orc_forward_project(g, col_ptr, idx, val, slice_index, x, yhat);
+    double c = orc_cost_state(g, raw, weights, penalty, lambda, delta, x, yhat, 0, sh);
+    free(yhat);
+    free(sh);
     return c;
 }
 

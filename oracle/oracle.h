@@ -90,6 +90,31 @@ int64_t orc_icd_trace(const ctbb_geom *g, const int64_t *col_ptr,
                       double *yhat, double *step_cost, int64_t max_costs,
                       int64_t *n_steps, int64_t *shared);
 
+/* orc_icd_trace with the penalty neighbourhood: 8 (the reference's flat
+ * in-plane stencil) or 26 (8 in-plane neighbours without wrap plus 9 in each
+ * adjacent slice through the slice rotation, see oracle.c). */
+int64_t orc_icd_nb(const ctbb_geom *g, const int64_t *col_ptr,
+                   const int32_t *idx, const float *val, const float *raw,
+                   const float *weights, const int64_t *slice_index,
+                   int penalty, double lambda, double delta, int neighbours,
+                   int64_t spacing, int64_t tile, int surrogate, int n_iter,
+                   double *x, double *yhat, double *step_cost, int64_t max_costs,
+                   int64_t *n_steps, int64_t *shared);
+/* Penalty step of voxel q (8 or 26 neighbours), before the clamp. */
+double orc_penalty_step(const ctbb_geom *g, const int64_t *slice_index,
+                        int neighbours, int penalty, int64_t q, double alpha,
+                        double beta, double lambda, double delta, const double *x);
+/* Cost with the penalty neighbourhood (8: orc_cost). */
+double orc_cost_nb(const ctbb_geom *g, const int64_t *col_ptr,
+                   const int32_t *idx, const float *val, const float *raw,
+                   const float *weights, const int64_t *slice_index,
+                   int penalty, double lambda, double delta, int neighbours,
+                   const double *x);
+/* The slice-to-slice rotation pi_k of the 26-neighbour penalty (inverse != 0:
+ * its inverse) applied to grid point (i, j). */
+void orc_rot26(const ctbb_geom *g, const int64_t *slice_index, int64_t k,
+               int inverse, int64_t i, int64_t j, int64_t *io, int64_t *jo);
+
 /* CPU baseline: the reference's OpenMP structure (oracle_cpu_ref.c). */
 int64_t orc_icd_ref_omp(const ctbb_geom *g, const int64_t *col_ptr,
                         const int32_t *idx, const float *val, const float *raw,

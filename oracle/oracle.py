@@ -59,6 +59,13 @@ def lib() -> C.CDLL:
             "orc_icd_trace": ([G, _P, _P, _P, _P, _P, _P, C.c_int, _f64, _f64, _i64,
                                _i64, C.c_int, C.c_int, _P, _P, _P, _i64, _P, _P], _i64),
             "orc_ssm_cols_count": ([G, _i64, _P, _P], _i64),
+            "orc_icd_nb": ([G, _P, _P, _P, _P, _P, _P, C.c_int, _f64, _f64, C.c_int, _i64,
+                            _i64, C.c_int, C.c_int, _P, _P, _P, _i64, _P, _P], _i64),
+            "orc_penalty_step": ([G, _P, C.c_int, C.c_int, _i64, _f64, _f64, _f64, _f64, _P],
+                                 _f64),
+            "orc_cost_nb": ([G, _P, _P, _P, _P, _P, _P, C.c_int, _f64, _f64, C.c_int, _P],
+                            _f64),
+            "orc_rot26": ([G, _P, _i64, C.c_int, _i64, _i64, _P, _P], None),
             "orc_ssm_cols_fill": ([G, _i64, _P, _P, _P, _P], None),
         }
         for n, (a, r) in sig.items():
@@ -126,7 +133,7 @@ def forward_project(geom, ssm, slice_index, x):
 
 
 def icd(geom, ssm, raw, weights, slice_index, penalty, lam, delta, spacing,
-        n_iter, x=None, yhat=None, tile=8, surrogate=False):
+        n_iter, x=None, yhat=None, tile=8, surrogate=False, neighbours=8):
     """Run n_iter oracle iterations; returns (x, yhat, updates).
     spacing 0: reference order; spacing >= 2: batched z-group order with the
     given tile (tile >= max(nx, ny): single-column batches), optionally with
@@ -140,10 +147,10 @@ def icd(geom, ssm, raw, weights, slice_index, penalty, lam, delta, spacing,
         yhat = np.array(yhat, np.float64)
     raw = np.ascontiguousarray(raw, np.float32)
     w = None if weights is None else np.ascontiguousarray(weights, np.float32)
-    n = lib().orc_icd_ex(gp(geom), _p(cp), _p(idx), _p(val), _p(raw), _p(w),
+    n = lib().orc_icd_nb(gp(geom), _p(cp), _p(idx), _p(val), _p(raw), _p(w),
                          _p(np.ascontiguousarray(slice_index, np.int64)), penalty, lam,
-                         delta, spacing, tile, int(surrogate), n_iter, _p(x),
-                         _p(yhat))
+                         delta, neighbours, spacing, tile, int(surrogate), n_iter, _p(x),
+                         _p(yhat), None, 0, None, None)
     return x, yhat, n
 
 
@@ -185,13 +192,26 @@ def icd_ref_omp(geom, ssm, raw, weights, slice_index, penalty, lam, delta,
     return x, yhat, n
 
 
-def cost(geom, ssm, raw, weights, slice_index, penalty, lam, delta, x):
+def cost(geom, ssm, raw, weights, slice_index, penalty, lam, delta, x, neighbours=8):
     cp, idx, val = ssm
     raw = np.ascontiguousarray(raw, np.float32)
     w = None if weights is None else np.ascontiguousarray(weights, np.float32)
-    return lib().orc_cost(gp(geom), _p(cp), _p(idx), _p(val), _p(raw), _p(w),
-                          _p(np.ascontiguousarray(slice_index, np.int64)), penalty,
-                          lam, delta, _p(np.ascontiguousarray(x, np.float64)))
+    return lib().orc_cost_nb(gp(geom), _p(cp), _p(idx), _p(val), _p(raw), _p(w),
+                             _p(np.ascontiguousarray(slice_index, np.int64)), penalty,
+                             lam, delta, neighbours, _p(np.ascontiguousarray(x, np.float64)))
+
+
+def penalty_step(geom, slice_index, neighbours, penalty, q, alpha, beta, lam, delta, x):
+    return lib().orc_penalty_step(gp(geom), _p(np.ascontiguousarray(slice_index, np.int64)),
+                                  neighbours, penalty, q, alpha, beta, lam, delta,
+                                  _p(np.ascontiguousarray(x, np.float64)))
+
+
+def rot26(geom, slice_index, k, i, j, inverse=False):
+    io, jo = np.zeros(1, np.int64), np.zeros(1, np.int64)
+    lib().orc_rot26(gp(geom), _p(np.ascontiguousarray(slice_index, np.int64)), k,
+                    int(inverse), i, j, _p(io), _p(jo))
+    return int(io[0]), int(jo[0])
 
 
 def quadratic(geom, q, alpha, beta, lam, x):
