@@ -135,7 +135,8 @@ def lib() -> C.CDLL:
         "ctbb_cost": ([P, P, P, P, C.POINTER(IcdOpts), P, P], C.c_int),
         "ctbb_cost_noalloc": ([P, P, P, P, C.POINTER(IcdOpts), P, P, P], C.c_int),
         "ctbb_rotate_slices": ([C.POINTER(Geom), P, P, P, C.c_int], C.c_int),
-        "ctbb_penalty_steps": ([C.POINTER(Geom), C.POINTER(IcdOpts), C.c_int, P, _i64, P, P, P, P], C.c_int),
+        "ctbb_penalty_steps": ([C.POINTER(Geom), C.POINTER(IcdOpts), P, C.c_int, P, _i64, P, P,
+                                P, P], C.c_int),
         "ctbb_unet_create": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                               C.POINTER(P)], C.c_int),
         "ctbb_unet_free": ([P], C.c_int),
@@ -452,9 +453,12 @@ def rotate_slices(geom: Geom, tube_angles, slice_index, volume, direction: int) 
 
 
 def penalty_steps(geom: Geom, opts: IcdOpts, x, q, alpha, beta,
-                  volume_fp32: bool = False) -> np.ndarray:
+                  volume_fp32: bool = False, slice_index=None) -> np.ndarray:
     """Device penalty steps (ctbb_penalty_steps); volume_fp32: evaluate on x
-    rounded to FP32, as the sweep holds the volume."""
+    rounded to FP32, as the sweep holds the volume; slice_index: needed for
+    the 26-neighbour penalty."""
+    si = None if slice_index is None else np.ascontiguousarray(slice_index, np.int64)
+    _need(si, geom.nz, "slice_index")
     x = np.ascontiguousarray(x, np.float64)
     q = np.ascontiguousarray(q, np.int64)
     a = np.ascontiguousarray(alpha, np.float64)
@@ -463,8 +467,8 @@ def penalty_steps(geom: Geom, opts: IcdOpts, x, q, alpha, beta,
     _need(a, q.size, "alpha")
     _need(b, q.size, "beta")
     du = np.zeros(len(q), np.float64)
-    check(lib().ctbb_penalty_steps(C.byref(geom), C.byref(opts), int(volume_fp32), _p(x),
-                                   len(q), _p(q),
+    check(lib().ctbb_penalty_steps(C.byref(geom), C.byref(opts), _p(si), int(volume_fp32),
+                                   _p(x), len(q), _p(q),
                                    _p(a), _p(b), _p(du)))
     return du
 

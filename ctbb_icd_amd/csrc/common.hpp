@@ -109,6 +109,7 @@ struct ctbb_recon {
     void *e = nullptr;          // running error sinogram y - A x
     void *x = nullptr;          // volume, rotated frame
     int64_t *slice_off = nullptr;  // [nz] sample offsets
+    double *shear = nullptr;    // [2*(nz-1)] 26-neighbour slice rotations (or null)
     int32_t *cols = nullptr;    // FOV columns, grouped by batch
     int64_t *batch_ptr = nullptr;  // [nbatch+1] offsets into cols
     void *ew = nullptr;         // weighted error w e (weighted recons)

@@ -141,13 +141,188 @@ __device__ double edge_preserving_step(int64_t q, double alpha, double beta,
     return c;
 }
 
+// 26-neighbour penalty (no reference counterpart: the reference stencil is
+// 8 in-plane neighbours, src/penalties.cpp:39-59, "limited to 2D",
+// src/icd_iteration.cpp:140).  Voxel p of rotating slice k sits at
+// R(theta_k) p, so the point of slice k+1 above it is
+// R(theta_k - theta_{k+1}) p; that rotation is applied as three shears with
+// integer-rounded shifts (an exact bijection of the grid with an exact
+// inverse).  Neighbours: the 8 in-plane voxels (no wrap), the 3x3 block of
+// slice k+1 around pi_k(p) and the voxels of slice k-1 whose blocks hold p
+// (pi_{k-1}^-1(p - d)); weights 1/|offset| in voxel units.  26 fixed slots,
+// in-plane (dy, dx) order, then above, then below, unused slots masked.
+struct Nb26 {
+    const double *shear;      // [2*(nz-1)]: a_k = -tan(phi_k/2), b_k = sin(phi_k);
+                              // null: the reference's 8-neighbour stencil
+    int64_t nx, ny, nz;
+    double cx, cy;
+};
+
+__device__ __forceinline__ double rnd_half(double v) { return floor(v + 0.5); }
+
+__device__ __forceinline__ void rot26_fwd(const Nb26 &n, int64_t k, int64_t i, int64_t j,
+                                          int64_t &io, int64_t &jo)
+{
+    const double a = n.shear[2 * k], b = n.shear[2 * k + 1];
+    const double x = (double)i - n.cx, y = (double)j - n.cy;
+    const double x1 = x + rnd_half(a * y);
+    const double y2 = y + rnd_half(b * x1);
+    const double x3 = x1 + rnd_half(a * y2);
+    io = (int64_t)(x3 + n.cx);
+    jo = (int64_t)(y2 + n.cy);
+}
+
+__device__ __forceinline__ void rot26_inv(const Nb26 &n, int64_t k, int64_t i, int64_t j,
+                                          int64_t &io, int64_t &jo)
+{
+    const double a = n.shear[2 * k], b = n.shear[2 * k + 1];
+    const double x3 = (double)i - n.cx, y2 = (double)j - n.cy;
+    const double x1 = x3 - rnd_half(a * y2);
+    const double y1 = y2 - rnd_half(b * x1);
+    const double x = x1 - rnd_half(a * y1);
+    io = (int64_t)(x + n.cx);
+    jo = (int64_t)(y1 + n.cy);
+}
+
+__device__ __forceinline__ double w26(int s)
+{
+    // slots 0..7 in-plane ((dy, dx) order without the centre), 8..16 above,
+    // 17..25 below ((dy, dx) order)
+    if (s < 8)
+        return (s == 0 || s == 2 || s == 5 || s == 7) ? 1.0 / kSqrt2 : 1.0;
+    const int t = (s - 8) % 9, dx = t % 3 - 1, dy = t / 3 - 1;
+    return 1.0 / sqrt((double)(dx * dx + dy * dy + 1));
+}
+
+// Differences x_n - x_q of the 26 slots and the mask of the slots present.
+template <typename X>
+__device__ __forceinline__ uint32_t diffs26(const Nb26 &n, int64_t q, const X *x,
+                                            double (&diff)[26])
+{
+    const int64_t nxy = n.nx * n.ny, k = q / nxy, c = q - k * nxy;
+    const int64_t j = c / n.nx, i = c - j * n.nx;
+    const double xq = (double)x[q];
+    uint32_t used = 0;
+    int s = 0;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            if (!dx && !dy)
+                continue;
+            const int64_t ii = i + dx, jj = j + dy;
+            const bool in = ii >= 0 && ii < n.nx && jj >= 0 && jj < n.ny;
+            diff[s] = in ? (double)x[ii + n.nx * jj + nxy * k] - xq : 0.0;
+            used |= in ? (1u << s) : 0u;
+            s++;
+        }
+    int64_t iu = 0, ju = 0;
+    const bool up = k + 1 < n.nz;
+    if (up)
+        rot26_fwd(n, k, i, j, iu, ju);
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            const int64_t ii = iu + dx, jj = ju + dy;
+            const bool in = up && ii >= 0 && ii < n.nx && jj >= 0 && jj < n.ny;
+            diff[s] = in ? (double)x[ii + n.nx * jj + nxy * (k + 1)] - xq : 0.0;
+            used |= in ? (1u << s) : 0u;
+            s++;
+        }
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+            int64_t ii = -1, jj = -1;
+            if (k >= 1)
+                rot26_inv(n, k - 1, i - dx, j - dy, ii, jj);
+            const bool in = k >= 1 && ii >= 0 && ii < n.nx && jj >= 0 && jj < n.ny;
+            diff[s] = in ? (double)x[ii + n.nx * jj + nxy * (k - 1)] - xq : 0.0;
+            used |= in ? (1u << s) : 0u;
+            s++;
+        }
+    return used;
+}
+
+// Quadratic: the coordinate minimiser over the neighbours present.
+template <typename X>
+__device__ double quadratic26_step(const Nb26 &n, int64_t q, double alpha, double beta,
+                                   double lambda, const X *x)
+{
+    double diff[26];
+    const uint32_t used = diffs26(n, q, x, diff);
+    double sum = 0.0, W = 0.0;
+#pragma unroll
+    for (int s = 0; s < 26; s++)
+        if (used & (1u << s)) {
+            sum += w26(s) * diff[s];
+            W += w26(s);
+        }
+    return (beta + lambda * sum) / (alpha + lambda * W);
+}
+
+__device__ __forceinline__ double ep26_func(double c, double alpha, double beta,
+                                            double delta, double lambda,
+                                            const double (&diff)[26], uint32_t used)
+{
+    double sum = 0.0;
+#pragma unroll
+    for (int s = 0; s < 26; s++)
+        if (used & (1u << s)) {
+            const double u = c - diff[s];
+            sum += (u >= 0 ? 1.0 : -1.0) * w26(s) * (1.0 - 1.0 / (1.0 + fabs(u / delta)));
+        }
+    return 2 * alpha * c - 2 * beta + lambda * sum;
+}
+
+// Edge-preserving: the bisection of src/penalties.cpp:88-139 over the
+// 26-neighbourhood (f(a) carried, as in edge_preserving_step).
+template <typename X>
+__device__ double ep26_step(const Nb26 &n, int64_t q, double alpha, double beta,
+                            double lambda, double delta, const X *x)
+{
+    if (alpha == 0.0)
+        return 0.0;
+    double diff[26];
+    const uint32_t used = diffs26(n, q, x, diff);
+    double a = beta / alpha, b = beta / alpha;
+#pragma unroll
+    for (int s = 0; s < 26; s++)
+        if (used & (1u << s)) {
+            if (diff[s] < a) a = diff[s];
+            if (diff[s] > b) b = diff[s];
+        }
+    const double tol = 0.00000001;
+    double c = (a + b) / 2.0;
+    double f_a = ep26_func(a, alpha, beta, delta, lambda, diff, used);
+    for (int it = 0; it < 4096; it++) {
+        c = (a + b) / 2.0;
+        const double f_c = ep26_func(c, alpha, beta, delta, lambda, diff, used);
+        if ((f_c == 0) || (b - a) / 2 < tol)
+            return c;
+        const int sc = f_c < 0 ? -1 : 1, sa = f_a < 0 ? -1 : 1;
+        if (sc == sa) {
+            a = c;
+            f_a = f_c;
+        } else {
+            b = c;
+        }
+    }
+    return c;
+}
+
 template <typename X>
 __device__ __forceinline__ double penalty_step(int penalty, int64_t q,
                                                double alpha, double beta,
                                                double lambda, double delta,
                                                const X *x, int64_t nx,
-                                               int64_t n)
+                                               int64_t n, const Nb26 &nb)
 {
+    if (nb.shear)
+        return penalty == CTBB_PENALTY_QUADRATIC
+                   ? quadratic26_step(nb, q, alpha, beta, lambda, x)
+                   : ep26_step(nb, q, alpha, beta, lambda, delta, x);
     if (penalty == CTBB_PENALTY_QUADRATIC)
         return quadratic_step(q, alpha, beta, lambda, x, nx, n);
     return edge_preserving_step(q, alpha, beta, lambda, delta, x, nx, n);
@@ -160,12 +335,12 @@ __global__ void penalty_steps_kernel(int penalty, double lambda, double delta,
                                      const X *x, int64_t nx, int64_t nvox,
                                      int64_t n, const int64_t *q,
                                      const double *alpha, const double *beta,
-                                     double *du)
+                                     double *du, Nb26 nb)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
         du[i] = penalty_step(penalty, q[i], alpha[i], beta[i], lambda, delta,
-                             x, nx, nvox);
+                             x, nx, nvox, nb);
 }
 
 // ---------------------------------------------------------------------------
@@ -190,6 +365,7 @@ struct SweepArgs {
     void *du;                 // T [ng*nc]: updates of the step
     double *th2;              // [nvox] theta2 per voxel (cached)
     const int64_t *slice_off;
+    Nb26 nb;                  // penalty neighbourhood (nb.shear null: 8)
     const int32_t *cols;      // batch columns, concatenated
     int64_t spacing;
     int64_t nz, nx, nxy, nvox, data_size;
@@ -531,7 +707,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_solve_kernel(SweepArgs a,
     else
         th2 = a.th2[q];
     double du = penalty_step(a.penalty, q, th2, th1, a.lambda, a.delta, x, a.nx,
-                             a.nvox);
+                             a.nvox, a.nb);
     const double xq = (double)x[q];
     if (du + xq < 0)
         du = -xq;
@@ -972,29 +1148,55 @@ __global__ __launch_bounds__(kRedBlock) void error_sq_kernel(const T *e, const f
         partials[blockIdx.x] = s;
 }
 
-// Penalty part of the cost: sum over pairs (q, q+d), d in {1, nx-1, nx, nx+1}.
+// Penalty part of the cost.  8 neighbours (the reference's flat stencil):
+// pairs (q, q+d), d in {1, nx-1, nx, nx+1}, q+d < n.  26 neighbours: per
+// voxel the in-plane forward pairs (i+1, j), (i-1, j+1), (i, j+1),
+// (i+1, j+1) inside the slice, then the 3x3 block above through pi_k.
 template <typename T>
 __global__ __launch_bounds__(kRedBlock) void penalty_cost_kernel(
     const T *x, int64_t nx, int64_t n, int penalty, double delta,
-    double *partials)
+    double *partials, Nb26 nb)
 {
     __shared__ double sh[kRedBlock / 64];
     const int64_t dd[4] = {1, nx - 1, nx, nx + 1};
     const double wd[4] = {1.0, 1.0 / kSqrt2, 1.0, 1.0 / kSqrt2};
+    auto term = [&](double diff, double w) {
+        if (penalty == CTBB_PENALTY_QUADRATIC)
+            return w * diff * diff;
+        const double ad = fabs(diff);
+        return w * (ad - delta * log1p(ad / delta));
+    };
     double acc = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; q < n;
          q += (int64_t)gridDim.x * kRedBlock) {
         const double xq = (double)x[q];
-        for (int t = 0; t < 4; t++) {
-            if (q + dd[t] >= n)
-                continue;
-            const double diff = xq - (double)x[q + dd[t]];
-            if (penalty == CTBB_PENALTY_QUADRATIC) {
-                acc += wd[t] * diff * diff;
-            } else {
-                const double ad = fabs(diff);
-                acc += wd[t] * (ad - delta * log1p(ad / delta));
+        if (!nb.shear) {
+            for (int t = 0; t < 4; t++) {
+                if (q + dd[t] >= n)
+                    continue;
+                acc += term(xq - (double)x[q + dd[t]], wd[t]);
             }
+            continue;
+        }
+        const int64_t nxy = nb.nx * nb.ny, k = q / nxy, c = q - k * nxy;
+        const int64_t j = c / nb.nx, i = c - j * nb.nx;
+        const int ox[4] = {1, -1, 0, 1}, oy[4] = {0, 1, 1, 1};
+        for (int t = 0; t < 4; t++) {
+            const int64_t ii = i + ox[t], jj = j + oy[t];
+            if (ii >= 0 && ii < nb.nx && jj >= 0 && jj < nb.ny)
+                acc += term(xq - (double)x[ii + nb.nx * jj + nxy * k],
+                            (ox[t] && oy[t]) ? 1.0 / kSqrt2 : 1.0);
+        }
+        if (k + 1 < nb.nz) {
+            int64_t iu, ju;
+            rot26_fwd(nb, k, i, j, iu, ju);
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int64_t ii = iu + dx, jj = ju + dy;
+                    if (ii >= 0 && ii < nb.nx && jj >= 0 && jj < nb.ny)
+                        acc += term(xq - (double)x[ii + nb.nx * jj + nxy * (k + 1)],
+                                    1.0 / sqrt((double)(dx * dx + dy * dy + 1)));
+                }
         }
     }
     const double s = block_sum_f64(acc, sh);
@@ -1227,6 +1429,64 @@ __global__ __launch_bounds__(kSweepBlock) void project_gather_kernel(
     }
 }
 
+// Shear parameters of the slice-to-slice rotations of the 26-neighbour
+// penalty: phi_k = theta_k - theta_{k+1} = -(c_{k+1} - c_k) * dtheta for
+// central views c_k (the tube angle advances by dtheta per view).
+std::vector<double> shear_table(const ctbb_geom *g, const int64_t *slice_index)
+{
+    std::vector<double> sh(2 * std::max<int64_t>(1, g->nz - 1), 0.0);
+    for (int64_t k = 0; k + 1 < g->nz; k++) {
+        const double phi = -(double)(slice_index[k + 1] - slice_index[k]) *
+                           g->tube_angle_increment;
+        sh[2 * k] = -tan(phi / 2.0);
+        sh[2 * k + 1] = sin(phi);
+    }
+    return sh;
+}
+
+Nb26 make_nb(const ctbb_geom *g, const double *d_shear)
+{
+    Nb26 n;
+    n.shear = d_shear;
+    n.nx = g->nx;
+    n.ny = g->ny;
+    n.nz = g->nz;
+    n.cx = g->center_voxel_x;
+    n.cy = g->center_voxel_y;
+    return n;
+}
+
+// neighbours must be 8 (the reference's stencil) or 26 (square slices).
+int check_neighbours(const ctbb_geom *g, int32_t neighbours)
+{
+    if (neighbours != 8 && neighbours != 26)
+        return fail(CTBB_E_ARG, "neighbours must be 8 or 26");
+    if (neighbours == 26 && (g->nx != g->ny || g->voxel_size_x != g->voxel_size_y))
+        return fail(CTBB_E_UNSUPPORTED, "the 26-neighbour penalty needs square slices "
+                                        "of square voxels");
+    return CTBB_OK;
+}
+
+// Upload the shear table of slice_index (26 neighbours) or leave null.
+int upload_shear(const ctbb_geom *g, int32_t neighbours, const int64_t *slice_index,
+                 double **d_shear)
+{
+    *d_shear = nullptr;
+    if (neighbours != 26)
+        return CTBB_OK;
+    if (!slice_index)
+        return fail(CTBB_E_ARG, "the 26-neighbour penalty needs the slice table");
+    const std::vector<double> sh = shear_table(g, slice_index);
+    CTBB_TRY(dev_alloc_n(d_shear, sh.size(), "slice rotation shears"));
+    hipError_t e = hipMemcpy(*d_shear, sh.data(), sh.size() * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(*d_shear);
+        *d_shear = nullptr;
+        return fail(CTBB_E_HIP, std::string("shear table: ") + hipGetErrorString(e));
+    }
+    return CTBB_OK;
+}
+
 int g_num_cu = 0;
 
 int num_cu()
@@ -1435,11 +1695,13 @@ int recon_penalty_term(ctbb_recon *r, double *pen)
     if (r->fp64)
         hipLaunchKernelGGL(penalty_cost_kernel<double>, dim3(g), dim3(kRedBlock), 0,
                            r->stream, static_cast<const double *>(r->x), r->g.nx, r->nvox,
-                           r->opts.penalty, r->opts.delta, r->partials);
+                           r->opts.penalty, r->opts.delta, r->partials,
+                           make_nb(&r->g, r->shear));
     else
         hipLaunchKernelGGL(penalty_cost_kernel<float>, dim3(g), dim3(kRedBlock), 0,
                            r->stream, static_cast<const float *>(r->x), r->g.nx, r->nvox,
-                           r->opts.penalty, r->opts.delta, r->partials);
+                           r->opts.penalty, r->opts.delta, r->partials,
+                           make_nb(&r->g, r->shear));
     CTBB_HIP(hipGetLastError());
     return sum_partials(r->partials, g, r->stream, pen);
 }
@@ -1498,9 +1760,9 @@ extern "C" int ctbb_forward_project(const ctbb_ssm *m,
 }
 
 extern "C" int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *o,
-                                  int volume_fp32, const double *x, int64_t n,
-                                  const int64_t *q, const double *alpha,
-                                  const double *beta, double *du)
+                                  const int64_t *slice_index, int volume_fp32,
+                                  const double *x, int64_t n, const int64_t *q,
+                                  const double *alpha, const double *beta, double *du)
 {
     CTBB_TRY(check_geom(g));
     if (!o || !x || !q || !alpha || !beta || !du || n < 0)
@@ -1508,9 +1770,7 @@ extern "C" int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *o,
     if (o->penalty != CTBB_PENALTY_QUADRATIC &&
         o->penalty != CTBB_PENALTY_EDGE_PRESERVING)
         return fail(CTBB_E_ARG, "unknown penalty");
-    if (o->neighbours != 8)
-        return fail(CTBB_E_UNSUPPORTED, "only the 8-neighbour stencil is "
-                                        "available in ctbb_penalty_steps");
+    CTBB_TRY(check_neighbours(g, o->neighbours));
     const int64_t nvox = g->nx * g->ny * g->nz;
     for (int64_t i = 0; i < n; i++)
         if (q[i] < 0 || q[i] >= nvox)
@@ -1518,15 +1778,19 @@ extern "C" int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *o,
     if (n == 0)
         return CTBB_OK;
     double *d_x = nullptr, *d_a = nullptr, *d_b = nullptr, *d_du = nullptr;
+    double *d_sh = nullptr;
     int64_t *d_q = nullptr;
     int rc = CTBB_OK;
+    if ((rc = upload_shear(g, o->neighbours, slice_index, &d_sh)))
+        return rc;
     if ((rc = dev_alloc_n(&d_x, nvox, "volume")) ||
         (rc = dev_alloc_n(&d_a, n, "alpha")) || (rc = dev_alloc_n(&d_b, n, "beta")) ||
         (rc = dev_alloc_n(&d_du, n, "du")) || (rc = dev_alloc_n(&d_q, n, "q"))) {
         (void)hipFree(d_x); (void)hipFree(d_a); (void)hipFree(d_b);
-        (void)hipFree(d_du); (void)hipFree(d_q);
+        (void)hipFree(d_du); (void)hipFree(d_q); (void)hipFree(d_sh);
         return rc;
     }
+    const Nb26 nb = make_nb(g, d_sh);
     hipError_t e = hipMemcpy(d_x, x, nvox * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_a, alpha, n * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_b, beta, n * 8, hipMemcpyHostToDevice);
@@ -1534,7 +1798,7 @@ extern "C" int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *o,
     if (e == hipSuccess && volume_fp32 == 0) {
         hipLaunchKernelGGL(penalty_steps_kernel<double>, dim3((n + 255) / 256), dim3(256),
                            0, 0, o->penalty, o->lambda, o->delta, d_x, g->nx,
-                           nvox, n, d_q, d_a, d_b, d_du);
+                           nvox, n, d_q, d_a, d_b, d_du, nb);
         e = hipGetLastError();
     } else if (e == hipSuccess) {
         std::vector<float> xf(nvox);
@@ -1546,7 +1810,7 @@ extern "C" int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *o,
             if (e == hipSuccess) {
                 hipLaunchKernelGGL(penalty_steps_kernel<float>, dim3((n + 255) / 256),
                                    dim3(256), 0, 0, o->penalty, o->lambda, o->delta,
-                                   d_xf, g->nx, nvox, n, d_q, d_a, d_b, d_du);
+                                   d_xf, g->nx, nvox, n, d_q, d_a, d_b, d_du, nb);
                 e = hipGetLastError();
             }
             (void)hipFree(d_xf);
@@ -1557,7 +1821,7 @@ extern "C" int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *o,
     if (e == hipSuccess)
         e = hipMemcpy(du, d_du, n * 8, hipMemcpyDeviceToHost);
     (void)hipFree(d_x); (void)hipFree(d_a); (void)hipFree(d_b);
-    (void)hipFree(d_du); (void)hipFree(d_q);
+    (void)hipFree(d_du); (void)hipFree(d_q); (void)hipFree(d_sh);
     CTBB_HIP(e);
     return CTBB_OK;
 }
@@ -1585,6 +1849,7 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
     if (r->part1) (void)hipFree(r->part1);
     if (r->ssum) (void)hipFree(r->ssum);
     if (r->du_m1) (void)hipFree(r->du_m1);
+    if (r->shear) (void)hipFree(r->shear);
     if (r->du_p1) (void)hipFree(r->du_p1);
     for (hipEvent_t ev : r->ev_sweep)
         if (ev) (void)hipEventDestroy(ev);
@@ -1971,8 +2236,7 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
         opts->penalty != CTBB_PENALTY_EDGE_PRESERVING)
         return fail(CTBB_E_ARG, "unknown penalty (valid: quadratic, "
                                 "edge-preserving)");
-    if (opts->neighbours != 8)
-        return fail(CTBB_E_UNSUPPORTED, "neighbours must be 8");
+    CTBB_TRY(check_neighbours(&m->g, opts->neighbours));
     if (opts->lambda < 0 || (opts->penalty == CTBB_PENALTY_EDGE_PRESERVING &&
                              !(opts->delta > 0)))
         return fail(CTBB_E_ARG, "need lambda >= 0 and delta > 0");
@@ -2104,6 +2368,8 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     }
     CTBB_HIP_R(hipMemset(r->x, 0, r->nvox * r->sb));
     CTBB_HIP_R(hipMemcpy(r->slice_off, off.data(), g->nz * 8, hipMemcpyHostToDevice));
+    if ((rc = upload_shear(g, opts->neighbours, slice_index, &r->shear)))
+        return bail(rc);
     r->slice_off_h = off;
     if (r->ncols_fov)
         CTBB_HIP_R(hipMemcpy(r->cols, cols.data(), r->ncols_fov * 4,
@@ -2307,6 +2573,7 @@ extern "C" int ctbb_cost(const ctbb_ssm *m, const float *raw, const float *weigh
         opts->penalty != CTBB_PENALTY_EDGE_PRESERVING)
         return fail(CTBB_E_ARG, "unknown penalty (valid: quadratic, edge-preserving)");
     const ctbb_geom *g = &m->g;
+    CTBB_TRY(check_neighbours(g, opts->neighbours));
     const int64_t nvox = g->nx * g->ny * g->nz, ds = data_size_of(g);
     for (int64_t k = 0; k < g->nz; k++)
         if (slice_index[k] < 0 || slice_index[k] >= g->readings)
@@ -2319,12 +2586,12 @@ extern "C" int ctbb_cost(const ctbb_ssm *m, const float *raw, const float *weigh
     float *d_x = nullptr, *d_y = nullptr, *d_w = nullptr;
     int64_t *d_sih = nullptr;
     int32_t *d_klo = nullptr, *d_khi = nullptr;
-    double *d_part = nullptr;
+    double *d_part = nullptr, *d_sh = nullptr;
     const int64_t np = std::max<int64_t>(project_grid(g), red_grid(nvox));
     int rc = CTBB_OK;
     auto cleanup = [&]() {
         for (void *p : {(void *)d_x, (void *)d_y, (void *)d_w, (void *)d_sih, (void *)d_klo,
-                        (void *)d_khi, (void *)d_part})
+                        (void *)d_khi, (void *)d_part, (void *)d_sh})
             if (p) (void)hipFree(p);
     };
     if ((rc = dev_alloc_n(&d_x, nvox, "volume")) ||
@@ -2333,7 +2600,8 @@ extern "C" int ctbb_cost(const ctbb_ssm *m, const float *raw, const float *weigh
         (rc = dev_alloc_n(&d_sih, g->nz, "slice view offsets")) ||
         (rc = dev_alloc_n(&d_klo, g->readings, "view slice ranges")) ||
         (rc = dev_alloc_n(&d_khi, g->readings, "view slice ranges")) ||
-        (rc = dev_alloc_n(&d_part, np, "reduction partials"))) {
+        (rc = dev_alloc_n(&d_part, np, "reduction partials")) ||
+        (rc = upload_shear(g, opts->neighbours, slice_index, &d_sh))) {
         cleanup();
         return rc;
     }
@@ -2352,7 +2620,8 @@ extern "C" int ctbb_cost(const ctbb_ssm *m, const float *raw, const float *weigh
         if (rc == CTBB_OK) {
             const int gp = red_grid(nvox);
             hipLaunchKernelGGL(penalty_cost_kernel<float>, dim3(gp), dim3(kRedBlock), 0, 0, d_x,
-                               g->nx, nvox, opts->penalty, opts->delta, d_part);
+                               g->nx, nvox, opts->penalty, opts->delta, d_part,
+                               make_nb(g, d_sh));
             e = hipGetLastError();
             if (e == hipSuccess)
                 rc = sum_partials(d_part, gp, 0, &pen);
@@ -2464,6 +2733,9 @@ extern "C" int ctbb_cost_noalloc(const ctbb_ssm *m, const float *raw,
     if (opts->penalty != CTBB_PENALTY_QUADRATIC &&
         opts->penalty != CTBB_PENALTY_EDGE_PRESERVING)
         return fail(CTBB_E_ARG, "unknown penalty (valid: quadratic, edge-preserving)");
+    if (opts->neighbours != 8)
+        return fail(CTBB_E_UNSUPPORTED, "the allocation-free cost has the 8-neighbour "
+                                        "penalty only (use ctbb_cost)");
     const ctbb_geom *g = &m->g;
     const int64_t nvox = g->nx * g->ny * g->nz;
     const int64_t nproj = project_grid(g), npen = red_grid(nvox);
@@ -2502,7 +2774,8 @@ extern "C" int ctbb_cost_noalloc(const ctbb_ssm *m, const float *raw,
     CTBB_TRY((launch_project<float, 1, float>(m, volume, d_sih, d_klo, d_khi, nullptr, raw, weights,
                                        nullptr, nullptr, d_pd, st)));
     hipLaunchKernelGGL(penalty_cost_kernel<float>, dim3((unsigned)npen), dim3(kRedBlock), 0, st,
-                       volume, g->nx, nvox, opts->penalty, opts->delta, d_pp);
+                       volume, g->nx, nvox, opts->penalty, opts->delta, d_pp,
+                       make_nb(g, nullptr));
     hipLaunchKernelGGL(noalloc_finish_kernel, dim3(1), dim3(64), 0, st, d_pd, nproj, d_pp,
                        npen, opts->penalty, opts->lambda, cost);
     CTBB_HIP(hipGetLastError());
@@ -2561,6 +2834,7 @@ static int recon_iterate_t(ctbb_recon *r, int n, ctbb_icd_stats *stats)
     a.w = r->w;
     a.x = r->x;
     a.slice_off = r->slice_off;
+    a.nb = make_nb(&r->g, r->shear);
     a.cols = r->cols;
     a.spacing = r->spacing;
     a.nz = r->g.nz;

@@ -65,8 +65,12 @@ class Pair:
 
 
 def generate_pairs(make_scan, seeds, ssm: SSM, n_early: int, n_target: int,
-                   log=print) -> list[Pair]:
-    """make_scan(seed) -> Scan of the shared geometry with that phantom."""
+                   log=print, target_start: str = "zero") -> list[Pair]:
+    """make_scan(seed) -> Scan of the shared geometry with that phantom.
+    Input: the image after n_early sweeps from zero.  Target: the image after
+    n_target sweeps from zero (target_start "zero", the early sweeps
+    included) or from the GPU FBP image ("fbp": closer to convergence for
+    the same number of sweeps)."""
     pairs = []
     for s in seeds:
         t = time.perf_counter()
@@ -75,12 +79,33 @@ def generate_pairs(make_scan, seeds, ssm: SSM, n_early: int, n_target: int,
         r = Recon(ssm, raw, w, scan.slice_index, scan.opts())
         r.iterate(n_early)
         early = to_fixed(scan, r.volume())
-        r.iterate(n_target - n_early)
+        if target_start == "fbp":
+            r.set_volume(np.maximum(ssm.fbp(raw, scan.slice_index), 0.0).astype(np.float32))
+            r.iterate(n_target)
+        else:
+            r.iterate(n_target - n_early)
         target = to_fixed(scan, r.volume())
         r.free()
         pairs.append(Pair(s, early, target))
         log(f"[warmstart] pair seed {s}: {time.perf_counter() - t:.1f} s")
     return pairs
+
+
+def save_params(path, net: UNet, meta: dict) -> None:
+    """The trained network as shipped: float16 parameters plus its shape and
+    training description (npz)."""
+    import json
+    np.savez_compressed(path, params=net.params().astype(np.float16),
+                        meta=json.dumps(dict(meta, levels=net.levels, base=net.base)))
+
+
+def load_params(path, max_batch: int, height: int, width: int) -> tuple[UNet, dict]:
+    import json
+    z = np.load(path, allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    net = UNet(meta["levels"], meta["base"], max_batch, height, width, seed=11)
+    net.set_params(z["params"].astype(np.float32))
+    return net, meta
 
 
 
@@ -271,6 +296,10 @@ def _main():
     ap.add_argument("--out", default="", help="JSON summary path")
     ap.add_argument("--params-out", default="", help="trained parameters (.npy)")
     ap.add_argument("--params-in", default="", help="parameters to evaluate (.npy)")
+    ap.add_argument("--target-start", choices=["zero", "fbp"], default="zero",
+                    help="training targets: sweeps from zero or from the FBP image")
+    ap.add_argument("--train-seed0", type=int, default=1000, help="first training phantom")
+    ap.add_argument("--save", default="", help="trained network as shipped (.npz)")
     a = ap.parse_args()
 
     set_verbose(False)
@@ -319,8 +348,9 @@ def _main():
         ssm = SSM.build(g)
         log(f"[warmstart] train on {tcfg}: {g.nx}x{g.ny}x{g.nz}, SSM in "
             f"{time.perf_counter() - t:.1f} s")
-        pairs = generate_pairs(lambda sd: make_scan(tcfg, sd), range(1000, 1000 + a.train),
-                               ssm, a.early, a.target_its, log)
+        pairs = generate_pairs(lambda sd: make_scan(tcfg, sd),
+                               range(a.train_seed0, a.train_seed0 + a.train),
+                               ssm, a.early, a.target_its, log, a.target_start)
         ssm.free()
         net = UNet(4, 32, a.batch, g.ny, g.nx, seed=11)
         t = time.perf_counter()
@@ -334,6 +364,13 @@ def _main():
                        unet_step_tflops=flops / (ms * 1e-3) / 1e12)
         if a.params_out:
             np.save(a.params_out, net.params())
+        if a.save:
+            save_params(a.save, net, dict(train_config=tcfg, phantoms=a.train,
+                                          seeds=[a.train_seed0, a.train_seed0 + a.train],
+                                          early=a.early, target_its=a.target_its,
+                                          target_start=a.target_start, epochs=a.epochs,
+                                          lr=a.lr, final_loss=hist[-1],
+                                          train_seconds=t_train))
         del pairs
     if a.phase in ("all", "eval"):
         s0 = make_scan(a.config, 1)

@@ -158,8 +158,21 @@ typedef struct ctbb_recon ctbb_recon;
 typedef struct ctbb_icd_opts {
     int32_t penalty;          /* CTBB_PENALTY_*                          */
     int32_t neighbours;       /* 8: the reference's in-plane stencil
-                                 (src/penalties.cpp:39-59); the only value
-                                 accepted                                */
+                                 (src/penalties.cpp:39-59, flat-index
+                                 offsets that wrap across rows and slices).
+                                 26 (no reference counterpart; square
+                                 slices): the 8 in-plane neighbours without
+                                 wrap and 9 in each adjacent slice, found
+                                 through the rotation between the slices'
+                                 frames -- voxel p of slice k meets the 3x3
+                                 block around pi_k(p) in slice k+1, pi_k the
+                                 three-shear integer rotation by the tube
+                                 angle between the slices' central views
+                                 (a bijection of the grid); weights
+                                 1/|offset| in voxel units.  The quadratic
+                                 step divides by the weights of the
+                                 neighbours present (the exact coordinate
+                                 minimiser). */
     double lambda;            /* regulariser weight (prm `lambda`)       */
     double delta;             /* edge-preserving delta (prm `delta`)     */
     int64_t group_spacing;    /* s; 0 = ctbb_ssm_group_spacing with conflict
@@ -252,9 +265,10 @@ int ctbb_recon_resync(ctbb_recon *r);
 /* Global cost of the current volume in FP64 (fresh A x, not the running e):
  * quadratic:  0.5 sum w e^2 + 0.5 lambda sum_pairs w_d (x_q - x_{q+d})^2
  * edge-pres.:     sum w e^2 +     lambda sum_pairs w_d rho(x_q - x_{q+d})
- * rho(t) = |t| - delta ln(1 + |t|/delta); pairs d in {+1, nx-1, nx, nx+1}
- * with q+d < nx*ny*nz (the forward half of the 8-neighbour flat-index
- * stencil, weights 1, 1/sqrt(2), 1, 1/sqrt(2)). */
+ * rho(t) = |t| - delta ln(1 + |t|/delta); 8 neighbours: pairs d in
+ * {+1, nx-1, nx, nx+1} with q+d < nx*ny*nz (the forward half of the flat
+ * stencil, weights 1, 1/sqrt(2), 1, 1/sqrt(2)); 26 neighbours: every pair
+ * of the neighbourhood once. */
 int ctbb_recon_cost(ctbb_recon *r, double *cost);
 
 /* Cost of the current volume from the kept FP32 error sinogram, without a
@@ -314,12 +328,14 @@ int ctbb_rotate_slices(const ctbb_geom *g, const double *tube_angles,
  * alpha[i], beta[i]; writes du[i] (before the positivity clamp).  The step
  * is the device function the sweep's solve kernel calls (quadratic closed
  * form; edge-preserving: the reference's bisection in FP64, tolerance 1e-8),
- * one thread per voxel.  volume_fp32 = 0: on x as given (FP64);
- * volume_fp32 = 1: on x rounded to FP32, the volume the sweep holds. */
+ * one thread per voxel, with the neighbourhood of opts->neighbours
+ * (slice_index[nz] is needed for 26, may be NULL for 8).  volume_fp32 = 0:
+ * on x as given (FP64); volume_fp32 = 1: on x rounded to FP32, the volume
+ * the sweep holds. */
 int ctbb_penalty_steps(const ctbb_geom *g, const ctbb_icd_opts *opts,
-                       int volume_fp32, const double *x, int64_t n,
-                       const int64_t *q, const double *alpha,
-                       const double *beta, double *du);
+                       const int64_t *slice_index, int volume_fp32,
+                       const double *x, int64_t n, const int64_t *q,
+                       const double *alpha, const double *beta, double *du);
 
 #ifdef __cplusplus
 }
