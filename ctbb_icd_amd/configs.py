@@ -17,6 +17,7 @@ the reference's key set.
                    edge-preserving lambda 5e3, delta 0.02/cm, I0 = 2e5
   config2          config1 with the FP16 SSM and an FP64 resync every 2 its
   config3          config1 geometry cut to 256x256x32 (1.4 mm pixels)
+  config3_slab     config3 on a 12-slice slab: training pairs for its warm start
   config4          512x512x256, 6 rotations, lung texture, I0 = 1e5
   config4_slab     config4's pixels and phantom model on a 32-slice slab
                    (2 rotations): training pairs for the warm start
@@ -109,6 +110,7 @@ CONFIGS["tiny_zffs"] = dict(CONFIGS["tiny_ffs"], Zffs=1, Phiffs=0, Nrows=16,
 CONFIGS["tiny_zphiffs"] = dict(CONFIGS["tiny_zffs"], Phiffs=1, Readings=576)
 CONFIGS["config3"] = dict(CONFIGS["config1"], Readings=4608, nx=256, ny=256,
                           z_start=-4.0, z_end=-2.05)
+CONFIGS["config3_slab"] = dict(CONFIGS["config3"], z_end=-3.27)
 CONFIGS["config4"] = dict(CONFIGS["config1"], Readings=13824, z_start=-16.5,
                           z_end=-1.11, synthetic_I0=1e5, synthetic_lung=4000,
                           synthetic_weights=1)

@@ -143,6 +143,123 @@ __global__ __launch_bounds__(256) void conv3x3_fwd_kernel(ConvIn in, const uint1
     }
 }
 
+// Tiled forward 3x3 convolution (also the input gradient, with flipped and
+// transposed weights).  A workgroup computes R = WPX image rows x 32 columns
+// of one image (32 pixels per wavefront row) x 32 * WCO output channels;
+// wavefront (wp, wc) owns row wp and channels 32 wc .. 32 wc + 31.  The K
+// dimension (9 taps x cin) is walked in stages of 16 input channels: the
+// (R + 2) x 34 input patch of the stage (zero outside the image) and its 9
+// x (32 WCO) x 16 weights are staged in LDS, and every wavefront issues 9
+// v_mfma_f32_32x32x16_bf16, one per tap, reading the patch shifted by the
+// tap.  The next stage's global loads are issued before the current stage
+// is computed.  Needs W % 32 == 0, H % R == 0, cin % 16 == 0, cout %
+// (32 WCO) == 0 and, with two input tensors, ca % 16 == 0.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+template <int WPX, int WCO>
+__global__ __launch_bounds__(64 * WPX * WCO) void conv3x3_fwd_tiled(
+    ConvIn in, const uint16_t *w, const float *bias, uint16_t *out, int N, int H, int W,
+    int cin, int cout, int relu)
+{
+    constexpr int R = WPX, PW = 34, NPIX = (R + 2) * PW, NT = 64 * WPX * WCO;
+    constexpr int CO = 32 * WCO;
+    constexpr int NA = NPIX * 2;              // 16-byte loads of the patch
+    constexpr int NB = 9 * CO * 2;            // 16-byte loads of the weights
+    constexpr int LA = (NA + NT - 1) / NT, LB = (NB + NT - 1) / NT;
+    __shared__ uint4 pa[NPIX * 2];            // [patch pixel][ci 0-7 | 8-15]
+    __shared__ uint4 pb[9 * CO * 2];          // [tap][co][ci 0-7 | 8-15]
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wp = wave % WPX, wc = wave / WPX;
+    // workgroup -> (image, row block, column block, channel block)
+    const int ncb = cout / CO, nxb = W / 32, nyb = H / R;
+    int64_t b = blockIdx.x;
+    const int cb = (int)(b % ncb);
+    b /= ncb;
+    const int xb = (int)(b % nxb);
+    b /= nxb;
+    const int yb = (int)(b % nyb);
+    const int n = (int)(b / nyb);
+    const int y0 = yb * R, x0 = xb * 32, co0 = cb * CO;
+    uint4 ra[LA], rb[LB];
+    const uint4 zero4 = {0u, 0u, 0u, 0u};
+    auto load_stage = [&](int ci0) {
+#pragma unroll
+        for (int t = 0; t < LA; t++) {
+            const int i = tid + t * NT;
+            uint4 v = zero4;
+            if (i < NA) {
+                const int pix = i >> 1, half = i & 1;
+                const int y = y0 - 1 + pix / PW, x = x0 - 1 + pix % PW;
+                const int c = ci0 + 8 * half;
+                if (y >= 0 && y < H && x >= 0 && x < W) {
+                    if (c < in.ca) {
+                        v = *reinterpret_cast<const uint4 *>(
+                            in.a + ((((int64_t)n * H + y) * W + x) * in.ca + c));
+                    } else if (in.up) {
+                        const int H2 = H >> 1, W2 = W >> 1;
+                        v = *reinterpret_cast<const uint4 *>(
+                            in.b + ((((int64_t)n * H2 + (y >> 1)) * W2 + (x >> 1)) * in.cb +
+                                    (c - in.ca)));
+                    } else {
+                        v = *reinterpret_cast<const uint4 *>(
+                            in.b + ((((int64_t)n * H + y) * W + x) * in.cb + (c - in.ca)));
+                    }
+                }
+            }
+            ra[t] = v;
+        }
+#pragma unroll
+        for (int t = 0; t < LB; t++) {
+            const int i = tid + t * NT;
+            uint4 v = zero4;
+            if (i < NB) {
+                const int half = i & 1, tc = i >> 1, tap = tc / CO, co = tc - tap * CO;
+                v = *reinterpret_cast<const uint4 *>(
+                    w + ((int64_t)tap * cout + co0 + co) * cin + ci0 + 8 * half);
+            }
+            rb[t] = v;
+        }
+    };
+    v16f acc = {0};
+    const int r = lane & 31, kh = lane >> 5;
+    load_stage(0);
+    for (int ci0 = 0; ci0 < cin; ci0 += 16) {
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < LA; t++)
+            if (tid + t * NT < NA)
+                pa[tid + t * NT] = ra[t];
+#pragma unroll
+        for (int t = 0; t < LB; t++)
+            if (tid + t * NT < NB)
+                pb[tid + t * NT] = rb[t];
+        __syncthreads();
+        if (ci0 + 16 < cin)
+            load_stage(ci0 + 16);
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            const uint4 av = pa[((wp + 1 + dy) * PW + (r + 1 + dx)) * 2 + kh];
+            const uint4 bv = pb[((tap * CO) + 32 * wc + r) * 2 + kh];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, av),
+                                                          __builtin_bit_cast(v8bf, bv), acc,
+                                                          0, 0, 0);
+        }
+    }
+    const int oc = co0 + 32 * wc + r;
+    const float bb = bias ? bias[oc] : 0.0f;
+    const int64_t rowbase = (((int64_t)n * H + y0 + wp) * W + x0);
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int px = 8 * (q >> 2) + 4 * kh + (q & 3);
+        float v = acc[q] + bb;
+        if (relu && v < 0.0f)
+            v = 0.0f;
+        out[(rowbase + px) * cout + oc] = f2bf(v);
+    }
+}
+
 // Weight gradient partials: part[chunk][tap][co][ci] = sum over the chunk's
 // pixels of dY[p][co] * in[p + o(tap)][ci].  Four wavefronts take quarters
 // of the chunk; their 32x32 tiles are added in LDS in wavefront order.
@@ -227,6 +344,111 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(ConvIn in, const uin
             const float s = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) +
                             red[3][q][lane];
             part[(((int64_t)ck * 9 + tap) * cout + oc) * cin + cc] = s;
+        }
+    }
+}
+
+// Tiled weight gradient: part[chunk][tap][co][ci] = sum over the chunk's
+// pixels of dY[p][co] in[p + o(tap)][ci], for a 32 x 32 (co, ci) block and
+// all 9 taps per workgroup.  A chunk is `crows` consecutive image rows of
+// the batch (stacked images, never split: H and crows are even); it is
+// walked in tiles of 2 rows x 32 columns.  Per tile the dY tile (64 pixels
+// x 32 channels) and the input patch (4 x 34 pixels x 32 channels) are
+// staged in LDS in their NHWC order with 16-byte loads (rows padded to 80
+// bytes: the two half-wavefronts' pixels fall in different banks).  The
+// MFMA's K dimension is the pixel: each lane gathers 8 consecutive pixels
+// of one channel from LDS.  Wavefront w accumulates taps w, w + 4 (and 8
+// for w = 0), reusing its dY operand across them.  Needs W % 32 == 0, H and
+// crows even, cout % 32 == 0, cin % 32 == 0 and, with two input tensors,
+// ca % 32 == 0.
+__global__ __launch_bounds__(256) void conv3x3_wgrad_tiled(ConvIn in, const uint16_t *dy,
+                                                           int N, int H, int W, int cin,
+                                                           int cout, int crows, int nco,
+                                                           int nci, float *part)
+{
+    constexpr int PW = 34, RS = 40;           // patch width, LDS row stride (u16)
+    __shared__ __align__(16) uint16_t dys[64 * RS];      // [tile pixel][co]
+    __shared__ __align__(16) uint16_t pch[4 * PW * RS];  // [patch pixel][ci]
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int64_t b = blockIdx.x;
+    const int ci_t = (int)(b % nci);
+    b /= nci;
+    const int co_t = (int)(b % nco);
+    const int64_t ck = b / nco;
+    const int co0 = co_t * 32, ci0 = ci_t * 32;
+    const int64_t rows_total = (int64_t)N * H;
+    const int64_t r0 = ck * crows, r1 = min(r0 + crows, rows_total);
+    const int ntap = wave == 0 ? 3 : 2;
+    v16f acc[3] = {{0}, {0}, {0}};
+    const int r = lane & 31, kh = lane >> 5;
+    const uint4 zero4 = {0u, 0u, 0u, 0u};
+    for (int64_t tr = r0; tr < r1; tr += 2) {
+        const int n = (int)(tr / H), y0 = (int)(tr - (int64_t)n * H);
+        for (int x0 = 0; x0 < W; x0 += 32) {
+            __syncthreads();
+            {   // dY tile: 64 pixels x 4 loads of 8 channels
+                const int px = tid >> 2, q = tid & 3;
+                const int y = y0 + (px >> 5), x = x0 + (px & 31);
+                const uint4 v = *reinterpret_cast<const uint4 *>(
+                    dy + (((int64_t)n * H + y) * W + x) * cout + co0 + 8 * q);
+                *reinterpret_cast<uint4 *>(&dys[px * RS + 8 * q]) = v;
+            }
+            for (int i = tid; i < 4 * PW * 4; i += 256) {   // patch: 136 pixels x 4 loads
+                const int pix = i >> 2, q = i & 3;
+                const int y = y0 - 1 + pix / PW, x = x0 - 1 + pix % PW;
+                const int c = ci0 + 8 * q;
+                uint4 v = zero4;
+                if (y >= 0 && y < H && x >= 0 && x < W) {
+                    if (c < in.ca) {
+                        v = *reinterpret_cast<const uint4 *>(
+                            in.a + ((((int64_t)n * H + y) * W + x) * in.ca + c));
+                    } else if (in.up) {
+                        const int H2 = H >> 1, W2 = W >> 1;
+                        v = *reinterpret_cast<const uint4 *>(
+                            in.b + ((((int64_t)n * H2 + (y >> 1)) * W2 + (x >> 1)) * in.cb +
+                                    (c - in.ca)));
+                    } else {
+                        v = *reinterpret_cast<const uint4 *>(
+                            in.b + ((((int64_t)n * H + y) * W + x) * in.cb + (c - in.ca)));
+                    }
+                }
+                *reinterpret_cast<uint4 *>(&pch[pix * RS + 8 * q]) = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s4 = 0; s4 < 4; s4++) {
+                // K block of 16 tile pixels; this lane's 8: p = 16 s4 + 8 kh + j
+                const int pb = 16 * s4 + 8 * kh, ty = pb >> 5, tx = pb & 31;
+                v8bf av;
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    av[j] = __builtin_bit_cast(__bf16, dys[(pb + j) * RS + r]);
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    if (t >= ntap)
+                        break;
+                    const int tap = wave + 4 * t;
+                    const int dyo = tap / 3 - 1, dxo = tap % 3 - 1;
+                    const int base = (ty + 1 + dyo) * PW + (tx + 1 + dxo);
+                    v8bf bv;
+#pragma unroll
+                    for (int j = 0; j < 8; j++)
+                        bv[j] = __builtin_bit_cast(__bf16, pch[(base + j) * RS + r]);
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[t], 0, 0, 0);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        if (t >= ntap)
+            break;
+        const int tap = wave + 4 * t;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int co = 8 * (q >> 2) + 4 * kh + (q & 3);
+            part[(((int64_t)ck * 9 + tap) * cout + co0 + co) * cin + ci0 + r] = acc[t][q];
         }
     }
 }
@@ -535,15 +757,35 @@ int64_t pixels(const ctbb_unet *u, int batch, int level)
     return (int64_t)batch * (u->H >> level) * (u->W >> level);
 }
 
+// 3x3 convolution in -> out (cin -> cout channels, weights [9][cout][cin]):
+// the tiled kernel where the shape fits it, else the per-wavefront kernel.
+void launch_conv(ctbb_unet *u, const ConvIn &in, const uint16_t *w, const float *bias,
+                 uint16_t *out, int batch, int Hl, int Wl, int cin, int cout, int relu)
+{
+    const bool fits = Wl % 32 == 0 && cin % 16 == 0 && cout % 32 == 0 &&
+                      (!in.b || in.ca % 16 == 0);
+    const int64_t P = (int64_t)batch * Hl * Wl;
+    if (fits && cout % 64 == 0 && Hl % 2 == 0) {
+        const int64_t nwg = (int64_t)batch * (Hl / 2) * (Wl / 32) * (cout / 64);
+        hipLaunchKernelGGL((conv3x3_fwd_tiled<2, 2>), dim3((unsigned)nwg), dim3(256), 0,
+                           u->stream, in, w, bias, out, batch, Hl, Wl, cin, cout, relu);
+    } else if (fits && Hl % 4 == 0) {
+        const int64_t nwg = (int64_t)batch * (Hl / 4) * (Wl / 32) * (cout / 32);
+        hipLaunchKernelGGL((conv3x3_fwd_tiled<4, 1>), dim3((unsigned)nwg), dim3(256), 0,
+                           u->stream, in, w, bias, out, batch, Hl, Wl, cin, cout, relu);
+    } else {
+        const int nct = (cout + 31) / 32;
+        const int64_t tiles = ((P + 31) / 32) * nct;
+        hipLaunchKernelGGL(conv3x3_fwd_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256),
+                           0, u->stream, in, w, bias, out, batch, Hl, Wl, cin, cout, relu, nct);
+    }
+}
+
 int run_conv_fwd(ctbb_unet *u, const Conv &c, int batch, double *flops)
 {
     const int Hl = u->H >> c.level, Wl = u->W >> c.level;
     const int64_t P = (int64_t)batch * Hl * Wl;
-    const int nct = (c.cout + 31) / 32;
-    const int64_t tiles = ((P + 31) / 32) * nct;
-    hipLaunchKernelGGL(conv3x3_fwd_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0,
-                       u->stream, c.in, c.wbf, u->params + c.boff, c.out, batch, Hl, Wl,
-                       c.cin, c.cout, 1, nct);
+    launch_conv(u, c.in, c.wbf, u->params + c.boff, c.out, batch, Hl, Wl, c.cin, c.cout, 1);
     *flops += 2.0 * P * 9.0 * c.cin * c.cout;
     CTBB_HIP(hipGetLastError());
     return CTBB_OK;
@@ -555,11 +797,7 @@ int run_conv_dgrad(ctbb_unet *u, const Conv &c, int batch, uint16_t *dst, double
     const int Hl = u->H >> c.level, Wl = u->W >> c.level;
     const int64_t P = (int64_t)batch * Hl * Wl;
     ConvIn gi{c.dout, c.cout, nullptr, 0, 0};
-    const int nct = (c.cin + 31) / 32;
-    const int64_t tiles = ((P + 31) / 32) * nct;
-    hipLaunchKernelGGL(conv3x3_fwd_kernel, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0,
-                       u->stream, gi, c.wt, nullptr, dst, batch, Hl, Wl, c.cout, c.cin, 0,
-                       nct);
+    launch_conv(u, gi, c.wt, nullptr, dst, batch, Hl, Wl, c.cout, c.cin, 0);
     *flops += 2.0 * P * 9.0 * c.cin * c.cout;
     CTBB_HIP(hipGetLastError());
     return CTBB_OK;
@@ -567,18 +805,44 @@ int run_conv_dgrad(ctbb_unet *u, const Conv &c, int batch, uint16_t *dst, double
 
 constexpr int64_t kWgradChunk = 4096;
 
+// The tiled weight-gradient kernel applies (image rows of a multiple of 32
+// pixels, 32-channel blocks) and its chunk height: whole rows, about
+// kWgradChunk pixels, even.
+bool wgrad_tiled(const Conv &c, int Hl, int Wl, int *crows)
+{
+    *crows = std::max<int>(2, (int)(kWgradChunk / std::max(1, Wl)) & ~1);
+    return Wl % 32 == 0 && Hl % 2 == 0 && c.cout % 32 == 0 && c.cin % 32 == 0 &&
+           (!c.in.b || c.in.ca % 32 == 0);
+}
+
+// Chunks (partial-sum slices) of one weight-gradient launch.
+int64_t wgrad_chunks(const Conv &c, int batch, int Hl, int Wl)
+{
+    int crows = 0;
+    if (wgrad_tiled(c, Hl, Wl, &crows))
+        return ((int64_t)batch * Hl + crows - 1) / crows;
+    return ((int64_t)batch * Hl * Wl + kWgradChunk - 1) / kWgradChunk;
+}
+
 int run_conv_wgrad(ctbb_unet *u, const Conv &c, int batch, double *flops)
 {
     const int Hl = u->H >> c.level, Wl = u->W >> c.level;
     const int64_t P = (int64_t)batch * Hl * Wl;
-    const int64_t nchunk = (P + kWgradChunk - 1) / kWgradChunk;
     const int nco = (c.cout + 31) / 32, nci = (c.cin + 31) / 32;
     const int64_t n = 9LL * c.cout * c.cin;
+    int crows = 0;
+    const bool tiled = wgrad_tiled(c, Hl, Wl, &crows);
+    const int64_t nchunk = wgrad_chunks(c, batch, Hl, Wl);
     if (nchunk * n > u->wpart_n)
         return fail(CTBB_E_STATE, "weight-gradient scratch too small");
-    hipLaunchKernelGGL(conv3x3_wgrad_kernel, dim3((unsigned)(nchunk * 9 * nco * nci)),
-                       dim3(256), 0, u->stream, c.in, c.dout, batch, Hl, Wl, c.cin, c.cout,
-                       kWgradChunk, nco, nci, u->wpart);
+    if (tiled)
+        hipLaunchKernelGGL(conv3x3_wgrad_tiled, dim3((unsigned)(nchunk * nco * nci)), dim3(256),
+                           0, u->stream, c.in, c.dout, batch, Hl, Wl, c.cin, c.cout, crows,
+                           nco, nci, u->wpart);
+    else
+        hipLaunchKernelGGL(conv3x3_wgrad_kernel, dim3((unsigned)(nchunk * 9 * nco * nci)),
+                           dim3(256), 0, u->stream, c.in, c.dout, batch, Hl, Wl, c.cin, c.cout,
+                           kWgradChunk, nco, nci, u->wpart);
     hipLaunchKernelGGL(sum_chunks_kernel, dim3(grid_for(n)), dim3(256), 0, u->stream,
                        u->wpart, nchunk, n, u->grad + c.woff);
     const int64_t nb = (P + kBiasChunk - 1) / kBiasChunk;
@@ -797,7 +1061,13 @@ extern "C" int ctbb_unet_create(int levels, int base_channels, int max_batch, in
     int64_t dx_max = 0, wp_max = 0;
     for (auto &c : u->convs) {
         dx_max = std::max(dx_max, pixels(u, max_batch, c.level) * c.cin);
-        const int64_t nch = (pixels(u, max_batch, c.level) + kWgradChunk - 1) / kWgradChunk;
+        // either kernel's chunk count (the inputs are not wired yet)
+        const int Hl = u->H >> c.level, Wl = u->W >> c.level;
+        int crows = 0;
+        wgrad_tiled(c, Hl, Wl, &crows);
+        const int64_t nch = std::max(((int64_t)max_batch * Hl + crows - 1) / crows,
+                                     (pixels(u, max_batch, c.level) + kWgradChunk - 1) /
+                                         kWgradChunk);
         wp_max = std::max<int64_t>(wp_max, nch * 9 * (int64_t)c.cin * c.cout);
     }
     u->wpart_n = wp_max;

@@ -47,6 +47,56 @@ def remaining_estimate(costs: list[float], k: int = 10) -> float:
     return float(d[-1] * ratio / (1.0 - ratio))
 
 
+def converged(scan: Scan, ssm: SSM, raw, w, rel_tol: float = 1e-9, pre_rel: float = 1e-7,
+              max_sweeps: int = 400, log=print) -> tuple[dict, np.ndarray]:
+    """Converged cost and volume of one scan with a matrix already built:
+    FP32 sweeps from the FBP image until a sweep lowers the running cost by
+    less than pre_rel (the FP32 state's floor is near 1e-7), then the FP64
+    state from that image until a sweep lowers the FP64 cost by less than
+    rel_tol.  The minimiser is unique, so the FP32 leg only shortens the
+    FP64 one.  Returns (summary, FP64 volume, rotated frame)."""
+    t0 = time.perf_counter()
+    x0 = np.maximum(ssm.fbp(raw, scan.slice_index), 0.0).astype(np.float32)
+    r = Recon(ssm, raw, w, scan.slice_index, scan.opts())
+    r.set_volume(x0)
+    prev, n32 = r.cost_running(), 0
+    while n32 < max_sweeps:
+        r.iterate(1)
+        n32 += 1
+        c = r.cost_running()
+        if n32 % 20 == 0:
+            log(f"[reference] FP32 sweep {n32}: running cost {c:.12e}")
+        if prev - c < pre_rel * c:
+            break
+        prev = c
+    x32 = r.volume64()
+    r.free()
+    r = Recon(ssm, raw, w, scan.slice_index, scan.opts(state_fp64=1))
+    r.set_volume64(x32)
+    costs = [r.cost()]
+    while len(costs) <= max_sweeps:
+        r.iterate(1)
+        costs.append(r.cost_running())
+        rel = (costs[-2] - costs[-1]) / costs[-1]
+        if (len(costs) - 1) % 10 == 0:
+            log(f"[reference] FP64 sweep {len(costs) - 1}: cost {costs[-1]:.12e}, "
+                f"decrease {rel:.2e}")
+        if 0 <= rel < rel_tol:
+            break
+    final = r.cost()
+    left = remaining_estimate(costs)
+    x = r.volume64()
+    r.free()
+    out = dict(cost=final, fp32_sweeps=n32, fp64_sweeps=len(costs) - 1,
+               last_decrease=(costs[-2] - costs[-1]) / costs[-1],
+               remaining_estimate=left, remaining_rel=left / final,
+               seconds=time.perf_counter() - t0)
+    log(f"[reference] converged cost {final:.12e} ({n32} FP32 + {out['fp64_sweeps']} FP64 "
+        f"sweeps, estimated remaining {out['remaining_rel']:.1e} relative, "
+        f"{out['seconds']:.1f} s)")
+    return out, x
+
+
 def run(config: str, seed: int, minutes: float, rel_tol: float, log=print) -> dict:
     scan = Scan(configs.write_prm(config, Path("/tmp") / f"ref_{config}_{seed}.yaml",
                                   synthetic_seed=seed))

@@ -153,13 +153,18 @@ class Run:
     reached: bool = False
     costs: list = field(default_factory=list)
     accepted: bool | None = None    # warm starts: passed the cost check
+    running: list = field(default_factory=list)   # running cost after every sweep
+    volume: np.ndarray | None = None               # final volume (keep=True)
 
 
 def run_to_target(r: Recon, target: float, max_iter: int, t0: float, run: Run,
-                  log=print, cost_fn=None) -> Run:
+                  log=print, cost_fn=None, screen: float = 0.0, keep: bool = False) -> Run:
     """Iterate until the FP64 cost is <= target (checked after every
     iteration; the cost evaluation is not counted in the time).  cost_fn(r)
-    evaluates the cost (default: the reconstruction's own)."""
+    evaluates the cost (default: the reconstruction's own).  screen > 0:
+    the FP64 cost is evaluated only once the running cost (kept error
+    sinogram, no projection) is within target * (1 + screen).  keep: the
+    final volume is kept in run.volume."""
     cost_fn = cost_fn or (lambda rr: rr.cost())
     spent = time.perf_counter() - t0
     while run.iterations < max_iter:
@@ -167,30 +172,42 @@ def run_to_target(r: Recon, target: float, max_iter: int, t0: float, run: Run,
         r.iterate(1)
         spent += time.perf_counter() - t
         run.iterations += 1
+        if screen > 0:
+            run.running.append(r.cost_running())
+            if run.iterations % 10 == 0:
+                log(f"[warmstart] {run.start} iteration {run.iterations}: running cost "
+                    f"{run.running[-1]:.10e} ({spent:.2f} s)")
+            if run.running[-1] > target * (1 + screen):
+                continue
         c = cost_fn(r)
         run.costs.append(c)
-        log(f"[warmstart] {run.start} iteration {run.iterations}: cost {c:.10e} "
-            f"({spent:.2f} s)")
+        if screen <= 0:
+            log(f"[warmstart] {run.start} iteration {run.iterations}: cost {c:.10e} "
+                f"({spent:.2f} s)")
         if c <= target:
             run.reached = True
             break
     run.seconds = spent
+    if keep:
+        run.volume = r.volume64()
     log(f"[warmstart] {run.start}: {run.iterations} iterations, {run.seconds:.2f} s, "
         f"reached={run.reached}")
     return run
 
 
 def plain_run(scan: Scan, ssm: SSM, raw, w, target: float, max_iter: int,
-              log=print) -> Run:
+              log=print, screen: float = 0.0, keep: bool = False) -> Run:
     r = Recon(ssm, raw, w, scan.slice_index, scan.opts())
     try:
-        return run_to_target(r, target, max_iter, time.perf_counter(), Run("plain"), log)
+        return run_to_target(r, target, max_iter, time.perf_counter(), Run("plain"), log,
+                             screen=screen, keep=keep)
     finally:
         r.free()
 
 
 def warm_run(scan: Scan, ssm: SSM, raw, w, net: UNet, n_early: int, target: float,
-             max_iter: int, log=print, cost_fn=None) -> Run:
+             max_iter: int, log=print, cost_fn=None, screen: float = 0.0,
+             keep: bool = False) -> Run:
     """n_early plain iterations, the network, the cost check, then ICD to the
     target.  The time includes the early iterations, the rotations, the
     network, the projection of the start image and the two cost evaluations
@@ -230,15 +247,17 @@ def warm_run(scan: Scan, ssm: SSM, raw, w, net: UNet, n_early: int, target: floa
         if run.costs[0] <= target:
             run.reached = True
             run.seconds = spent
+            if keep:
+                run.volume = r.volume64()
             return run
         return run_to_target(r, target, max_iter, time.perf_counter() - spent, run, log,
-                             cost_fn)
+                             cost_fn, screen, keep)
     finally:
         r.free()
 
 
 def fbp_run(scan: Scan, ssm: SSM, raw, w, target: float, max_iter: int, log=print,
-            cost_fn=None) -> Run:
+            cost_fn=None, screen: float = 0.0, keep: bool = False) -> Run:
     """ICD from the GPU filtered backprojection start (ctbb_fbp, clamped at
     0); the time includes the FBP and the projection of the start image."""
     run = Run("fbp")
@@ -253,9 +272,11 @@ def fbp_run(scan: Scan, ssm: SSM, raw, w, target: float, max_iter: int, log=prin
         if run.costs[0] <= target:
             run.reached = True
             run.seconds = spent
+            if keep:
+                run.volume = r.volume64()
             return run
         return run_to_target(r, target, max_iter, time.perf_counter() - spent, run, log,
-                             cost_fn)
+                             cost_fn, screen, keep)
     finally:
         r.free()
 
@@ -300,6 +321,15 @@ def _main():
                     help="training targets: sweeps from zero or from the FBP image")
     ap.add_argument("--train-seed0", type=int, default=1000, help="first training phantom")
     ap.add_argument("--save", default="", help="trained network as shipped (.npz)")
+    ap.add_argument("--net", default="", help="network to evaluate (.npz from --save)")
+    ap.add_argument("--converged", action="store_true",
+                    help="targets from converged FP64 references (reference.converged) "
+                         "instead of a plain run of --ref-its sweeps; --ref-its caps the "
+                         "plain and warm runs")
+    ap.add_argument("--ref-rel", type=float, default=1e-9,
+                    help="converged references: stop when an FP64 sweep lowers the cost "
+                         "by less than this (relative)")
+    ap.add_argument("--test-seed0", type=int, default=2000, help="first held-out phantom")
     a = ap.parse_args()
 
     set_verbose(False)
@@ -383,12 +413,18 @@ def _main():
         if a.fp16_warm:
             ssm_w = SSM.build(g)
             ssm_w.to_fp16()
+        if a.net and net is None:
+            net, meta = load_params(a.net, a.batch, g.ny, g.nx)
+            summary.update(network=dict(meta, file=Path(a.net).name))
         if net is None or net.height != g.ny or net.width != g.nx:
             p = net.params() if net is not None else np.load(a.params_in)
             net = UNet(4, 32, a.batch, g.ny, g.nx, seed=11)
             net.set_params(p)
+        if a.converged:
+            _eval_converged(a, summary, make_scan, ssm, ssm_w, net, log, model_cost)
+            return
         results = []
-        for s in range(2000, 2000 + a.test):
+        for s in range(a.test_seed0, a.test_seed0 + a.test):
             scan = make_scan(a.config, s)
             raw, w = scan.synthesize(use_gpu=True)
             opts = scan.opts()
@@ -438,6 +474,79 @@ def _main():
             f"{wsec:.2f} s, speedup {ps / wsec:.2f}x")
     if a.out:
         Path(a.out).write_text(json.dumps(summary, indent=1))
+
+
+def _eval_converged(a, summary, make_scan, ssm, ssm_w, net, log, model_cost):
+    """Held-out phantoms a.test_seed0 ...: the converged FP64 reference
+    (reference.converged), then plain FP32 from zero and the learned start,
+    each to (1 + a.tol) x the converged cost; iterations, wall time and the
+    volume error in HU against the converged volume.  The summary (a.out) is
+    rewritten after every phantom."""
+    import json
+    from pathlib import Path
+    from . import reference
+    hu = MU_WATER / 1000.0
+    results = []
+    for s in range(a.test_seed0, a.test_seed0 + a.test):
+        scan = make_scan(a.config, s)
+        raw, w = scan.synthesize(use_gpu=True)
+        opts = scan.opts()
+        ref, x_ref = reference.converged(scan, ssm, raw, w, rel_tol=a.ref_rel, log=log)
+        target = ref["cost"] * (1 + a.tol)
+        cost_fn, slack = None, 1e-6
+        if a.fp16_warm:
+            def cost_fn(rr, raw=raw, w=w, scan=scan, opts=opts):
+                return model_cost(ssm, raw, w, scan.slice_index, opts, rr.volume())
+            slack = 1e-4
+        plain = plain_run(scan, ssm, raw, w, target, a.ref_its, log, screen=1e-6, keep=True)
+        warm = warm_run(scan, ssm_w, raw, w, net, a.early, target, a.ref_its, log, cost_fn,
+                        screen=slack, keep=True)
+
+        nxy, nz = scan.geom.nx * scan.geom.ny, scan.geom.nz
+
+        def rmse(v, ends=4):
+            """HU error against the converged volume: all voxels, and the
+            slices away from the slab ends (which converge slowest)."""
+            d = (v - x_ref).reshape(nz, nxy)
+            return dict(all=float(np.sqrt(np.mean(d ** 2)) / hu),
+                        interior=float(np.sqrt(np.mean(d[ends:nz - ends] ** 2)) / hu))
+        res = dict(seed=s, reference=ref, target=target,
+                   plain=dict(iterations=plain.iterations, seconds=plain.seconds,
+                              reached=plain.reached, cost=plain.costs[-1] if plain.costs
+                              else None, rmse_hu=rmse(plain.volume)),
+                   warm=dict(iterations=warm.iterations, seconds=warm.seconds,
+                             reached=warm.reached, accepted=warm.accepted,
+                             cost=warm.costs[-1] if warm.costs else None,
+                             rmse_hu=rmse(warm.volume)))
+        if a.fbp:
+            fr = fbp_run(scan, ssm_w, raw, w, target, a.ref_its, log, cost_fn, screen=slack,
+                         keep=True)
+            res["fbp"] = dict(iterations=fr.iterations, seconds=fr.seconds,
+                              reached=fr.reached, rmse_hu=rmse(fr.volume))
+        results.append(res)
+        log(f"[warmstart] seed {s}: converged {ref['cost']:.10e}; plain {plain.iterations} "
+            f"its / {plain.seconds:.2f} s ({res['plain']['rmse_hu']['interior']:.2f} HU), warm "
+            f"{warm.iterations} its / {warm.seconds:.2f} s "
+            f"({res['warm']['rmse_hu']['interior']:.2f} HU, "
+            f"{'accepted' if warm.accepted else 'rejected'})")
+        ps = float(np.mean([x["plain"]["seconds"] for x in results]))
+        wsec = float(np.mean([x["warm"]["seconds"] for x in results]))
+        summary.update(test=len(results), test_seed0=a.test_seed0, cap_iterations=a.ref_its,
+                       results=results,
+                       mean_plain_iterations=float(np.mean([x["plain"]["iterations"]
+                                                            for x in results])),
+                       mean_warm_iterations=float(np.mean([x["warm"]["iterations"]
+                                                           for x in results])),
+                       mean_plain_seconds=ps, mean_warm_seconds=wsec,
+                       speedup=ps / wsec if wsec > 0 else None,
+                       all_reached=all(x["plain"]["reached"] and x["warm"]["reached"]
+                                       for x in results))
+        if a.out:
+            Path(a.out).write_text(json.dumps(summary, indent=1))
+    log(f"[warmstart] mean over {len(results)}: plain {summary['mean_plain_iterations']:.1f} "
+        f"its / {summary['mean_plain_seconds']:.2f} s, warm "
+        f"{summary['mean_warm_iterations']:.1f} its / {summary['mean_warm_seconds']:.2f} s, "
+        f"speedup {summary['speedup']:.2f}x")
 
 
 if __name__ == "__main__":

@@ -29,7 +29,8 @@ def _tensor_slices(L, base):
     return out
 
 
-@pytest.mark.parametrize("L,base,B,H,W", [(3, 8, 2, 16, 24), (4, 32, 1, 32, 40)])
+@pytest.mark.parametrize("L,base,B,H,W", [(3, 8, 2, 16, 24), (4, 32, 1, 32, 40),
+                                          (4, 32, 2, 64, 64)])
 def test_forward_and_gradients_match_oracle(gpu, L, base, B, H, W):
     print(f"[unet] L={L} base={base} batch={B} {H}x{W}", flush=True)
     rng = np.random.default_rng(5)
