@@ -27,9 +27,20 @@
  *   orc_icd_ex           batch surrogate: no batched sweep raises the cost
  *                        where plain Jacobi curvature does; single-column
  *                        batches give the reference's curvature bit for bit.
- *   orc_column_overlaps  exact by construction (a sorted merge of a column's
- *                        samples with the same samples dv views later); it
- *                        checks the device's group spacing.
+ *   orc_column_overlaps  brute force: np.intersect1d of a column's sample set
+ *                        with the same set dv views later, every column and
+ *                        shift; it checks the device's group spacing.
+ *   orc_ssm_cols         equals the ray-driven build on its columns; in-plane
+ *                        flying focal spot: even/odd-view footprint centroids
+ *                        of central voxels 0.50 +- 0.02 channel apart (the
+ *                        interlace the deflection da exists for).
+ *   orc_icd_trace        every step of a default-schedule sweep on the
+ *                        clinical cone lowers the cost, with shared samples
+ *                        between members counted (> 0).
+ *   orc_*26              three-shear rotation a bijection near the true
+ *                        rotation; steps are exact coordinate minimisers of
+ *                        the 26-neighbour cost; constant image -> zero step;
+ *                        sweeps never raise the cost.
  *   orc_cost             gradient of the cost equals the update optimality
  *                        residual; finite differences.
  *   orc_rotate_slices    reference src/rotate_slices.cpp + src/interp.h

@@ -11,6 +11,8 @@
  * held in RAM (the reference re-reads it from disk each iteration,
  * src/icd_iteration.cpp:164-179; that I/O is not timed here).
  * Only the quadratic and edge-preserving 8-neighbour penalties are used.
+ * Pin: equals the sequential orc_icd in the reference order (spacing 0)
+ * within 1e-12 on tiny and tiny_ffs (tests/test_oracle_clinical.py).
  */
 #include "oracle.h"
 

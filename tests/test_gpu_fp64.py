@@ -75,10 +75,14 @@ def test_default_schedule_converges_to_reference_order_cost(gpu, make_scan, orc,
     surrogate), FP32 and FP64 state, run to convergence, against the
     reference order run to convergence (src/icd_iteration.cpp:157-185).
     tiny_ffs: the oracle's own reference order (raster columns, slices
-    ascending).  config0, where the oracle needs ~30 s per sweep: the FP64
-    state with single-column batches -- the reference's raster column
-    order, slices permuted inside a column into conflict-free groups,
-    checked against the oracle per sweep in the FP64 parity test above."""
+    ascending), run here.  config0, where the oracle needs ~30 s per sweep
+    and the GPU's single-column schedule minutes: the converged cost of the
+    FP64 state with single-column batches -- the reference's raster column
+    order, checked against the oracle per sweep in the FP64 parity test
+    above -- from tests/golden/config0_converged.json
+    (tests/golden/make_config0_converged.py), for the same inputs."""
+    import json
+    from pathlib import Path
     s = make_scan(name)
     g = s.geom
     raw, w = s.synthesize(use_gpu=False)
@@ -95,9 +99,14 @@ def test_default_schedule_converges_to_reference_order_cost(gpu, make_scan, orc,
                 break
             prev = c_ref
     else:
-        rr = gpu.Recon(m, raw, w, s.slice_index, s.opts(tile=1024, state_fp64=1))
-        c_ref, n_ref = _converge(rr, "single-column FP64", rel=1e-8)
-        rr.free()
+        fx = json.loads((Path(__file__).parent / "golden" / "config0_converged.json")
+                        .read_text())
+        fp = fx["inputs"]
+        assert float(np.sum(raw, dtype=np.float64)) == pytest.approx(fp["raw_sum"], rel=1e-12)
+        assert float(np.sum(np.square(raw, dtype=np.float64))) == pytest.approx(
+            fp["raw_sumsq"], rel=1e-12)
+        assert float(np.sum(w, dtype=np.float64)) == pytest.approx(fp["w_sum"], rel=1e-12)
+        c_ref, n_ref = fx["cost"], fx["sweeps"]
     out = {}
     for fp64 in (0, 1):
         r = gpu.Recon(m, raw, w, s.slice_index, s.opts(state_fp64=fp64))

@@ -49,3 +49,66 @@ def test_identity_network_is_accepted_and_reaches_target(ct, setup):
     # either way, but the run must reach the target
     assert run.reached
     assert run.accepted in (True, False)
+
+
+def test_shipped_network_gains_over_plain_at_equal_time(ct, make_scan):
+    """The shipped network (ctbb_icd_amd/data/unet_config4.npz, trained on
+    config4_slab phantoms 1000-1023: one-sweep image -> 50 sweeps from the
+    FBP image) on held-out phantom 2001 of the same scan model.  Its start is
+    accepted by the cost check and costs less than the one-sweep image it was
+    made from; after equal wall time -- the plain run from zero gets every
+    sweep that fits in the warm run's time, the network, rotations and
+    projection included -- the warm run's FP64 cost is below the plain
+    run's."""
+    import time
+    from pathlib import Path
+
+    from ctbb_icd_amd import warmstart as ws
+    ct.init(0)
+    seed = 2001
+    s = make_scan("config4_slab", synthetic_seed=seed)
+    g = s.geom
+    net, meta = ws.load_params(Path(ct.__file__).parent / "data" / "unet_config4.npz", 8,
+                               g.ny, g.nx)
+    assert meta["train_config"] == "config4_slab"
+    assert not (meta["seeds"][0] <= seed < meta["seeds"][1])     # held out
+    ssm = ct.SSM.build(g)
+    raw, w = s.synthesize(use_gpu=True)
+    n_after = 3
+
+    # warm: one plain sweep, the network, the cost check, then n_after sweeps
+    r = ct.Recon(ssm, raw, w, s.slice_index, s.opts())
+    r.iterate(1)                                   # untimed: first-sweep set-up
+    r.free()
+    r = ct.Recon(ssm, raw, w, s.slice_index, s.opts())
+    t0 = time.perf_counter()
+    r.iterate(1)
+    x1 = r.volume()
+    c1 = r.cost_running()
+    pred = ws.to_rotating(s, ws.predict_fixed(net, ws.to_fixed(s, x1)))
+    r.set_volume(pred)
+    c_start = r.cost_running()
+    r.iterate(n_after)
+    t_warm = time.perf_counter() - t0
+    c_warm = r.cost()
+    r.free()
+    assert c_start < c1, (c_start, c1)            # accepted, and lower
+
+    # plain from zero for the same wall time
+    r = ct.Recon(ssm, raw, w, s.slice_index, s.opts())
+    t0 = time.perf_counter()
+    n_plain = 0
+    while time.perf_counter() - t0 < t_warm:
+        r.iterate(1)
+        n_plain += 1
+    t_plain = time.perf_counter() - t0
+    c_plain = r.cost()
+    r.free()
+    ssm.free()
+    net.free()
+    print(f"[gain] seed {seed}: one-sweep cost {c1:.8e}, learned start {c_start:.8e}; "
+          f"warm 1+{n_after} sweeps in {t_warm:.2f} s -> {c_warm:.8e}; plain {n_plain} "
+          f"sweeps in {t_plain:.2f} s -> {c_plain:.8e} (rel gap "
+          f"{(c_plain - c_warm) / c_warm:.2e})", flush=True)
+    assert n_plain >= 1 + n_after
+    assert c_warm < c_plain
