@@ -378,16 +378,61 @@ struct SweepArgs {
 // runs are contiguous because keys arrive sorted.  After the call, the last
 // lane of each run holds the sum of its run; the order of additions is fixed
 // (Hillis-Steele), so the result is deterministic.
+// DPP lane moves: row_shr:n inside rows of 16 lanes (0 from outside the
+// row), row_bcast:15 / row_bcast:31 (lane 15 of each row to the next row /
+// lane 31 to rows 2 and 3), wave_shr:1 (lane l - 1; lane 0 keeps `old`).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_mov(float v)
+{
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xf,
+                                               false);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Inclusive sums over runs of equal keys (segmented scan): lane l adds the
+// values of lanes [start, l], start = the last head at or before l.  Four
+// in-row shifts, then the carries across rows (row 0 -> 1 and 2 -> 3, then
+// rows 0-1 -> 2 and 3); steps no lane needs (every run shorter) are
+// skipped, wave-uniformly.
 template <typename T>
 __device__ __forceinline__ T run_sum(T v, uint64_t head, int lane)
 {
     const uint64_t upto = head & (~0ull >> (63 - lane));
     const int start = 63 - __clzll((unsigned long long)upto);
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const T u = __shfl_up(v, off, 64);
-        if (lane - off >= start)
-            v += u;
+    const int dist = lane - start;
+    if (!__ballot(dist >= 1))
+        return v;
+    T t;
+    t = dpp_mov<0x111, 0xf>(v);
+    if (dist >= 1) v += t;
+    if (__ballot(dist >= 2)) {
+        t = dpp_mov<0x112, 0xf>(v);
+        if (dist >= 2) v += t;
+        if (__ballot(dist >= 4)) {
+            t = dpp_mov<0x114, 0xf>(v);
+            if (dist >= 4) v += t;
+            if (__ballot(dist >= 8)) {
+                t = dpp_mov<0x118, 0xf>(v);
+                if (dist >= 8) v += t;
+            }
+        }
+    }
+    // carries across rows: lanes whose run began in an earlier row
+    const int row0 = lane & ~15;
+    if (__ballot(start < row0)) {
+        t = dpp_mov<0x142, 0xa>(v);
+        if ((lane & 16) && start < row0) v += t;
+        t = dpp_mov<0x143, 0xc>(v);
+        if ((lane & 32) && start < 32) v += t;
     }
     return v;
 }
@@ -396,7 +441,7 @@ __device__ __forceinline__ T run_sum(T v, uint64_t head, int lane)
 // the tail, carry key -1) and run-end test for lane `lane`.
 __device__ __forceinline__ uint64_t run_heads(int key, int lane)
 {
-    const int prev = __shfl_up(key, 1, 64);
+    const int prev = __builtin_amdgcn_update_dpp(-2, key, 0x138, 0xf, 0xf, false);
     return __ballot(lane == 0 || prev != key);
 }
 
@@ -720,16 +765,21 @@ __global__ __launch_bounds__(kSweepBlock) void icd_solve_kernel(SweepArgs a,
 // channels (all rows) and adds up, in LDS, the updates of every voxel of the
 // step that reaches them: for a data view and a slice k of the group, the
 // SSM view i = view - (c_k - n_sm/2) and the batch's transposed entries of
-// that view (sorted by channel, row, then column slot) give the rays and
-// their column slots.  Wavefront w owns the channels [c0 + 8w, c0 + 8w + 8)
-// of every view of the workgroup: one (view, slice) item's entries for them
-// are contiguous and are taken in 64-entry chunks.  Lanes form du[slot] * a,
-// runs of equal (channel, row) are summed with shuffles and the last lane of
-// a run adds it into LDS.  The loads of kItemUnroll chunks are issued before
-// any of them is reduced, and chunks are applied in a fixed order by the one
-// wavefront that owns the channels: no LDS atomics, deterministic sums.  The workgroup then writes each changed
-// sample once; no two workgroups touch the same sample, so the scatter is
-// conflict-free.
+// that view (sorted by channel, row, then column slot; every run of one
+// sample padded to a multiple of four entries) give the rays and their
+// column slots.  Wavefront w owns the channels [c0 + 8w, c0 + 8w + 8) of
+// every view of the workgroup: one (view, slice) item's entries for them are
+// contiguous and are taken in chunks of 64 quads (each lane four entries of
+// one sample, two 16-byte loads).  Lanes form sum du[slot] * a over their
+// quad, runs of equal (channel, row) across lanes are summed with DPP lane
+// moves and the last lane of a run adds it into LDS.  The loads of
+// kItemUnroll chunks are issued before any of them is reduced, and chunks
+// are applied in a fixed order by the one wavefront that owns the channels:
+// no LDS atomics, deterministic sums.  The workgroup then writes each
+// changed sample once; no two workgroups touch the same sample, so the
+// scatter is conflict-free.  The du reads are gathers over the batch's
+// slots; slots follow the Z-order curve of the tile grid, so the columns
+// along a ray, whose entries make up a run, mostly share cache lines.
 // Tile and unroll sizes (overridable at build time for tuning sweeps).
 #ifndef CTBB_SCATTER_CH
 #define CTBB_SCATTER_CH 32
@@ -748,6 +798,10 @@ constexpr int kItemUnroll = CTBB_ITEM_UNROLL;
 // lkey = (ch % kScatterCh) * nr + row is the sample's place in the scatter
 // tile of its channel group (needs kScatterCh * nr <= 4096).
 constexpr uint32_t kSlotBits = 14, kLkeyBits = 12;
+// Runs of one sample in the transposed copy are padded to multiples of kQuad
+// entries (copies of the run's last entry with value 0), so each lane of the
+// scatter takes four entries of one sample with two 16-byte loads.
+constexpr int kQuad = 4;
 
 __device__ __forceinline__ uint32_t bt_slot(uint32_t p) { return p & ((1u << kSlotBits) - 1); }
 __device__ __forceinline__ uint32_t bt_lkey(uint32_t p) { return (p >> kSlotBits) & ((1u << kLkeyBits) - 1); }
@@ -830,8 +884,9 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
         if (u_l < nview && v_l < sa.ng && wc1 > wc0) {
             const int64_t i = view0 + u_l - bt.si_half[sa.j + a.spacing * v_l];
             if (i >= 0 && i < bt.nsm) {
-                lo = vci[i * bt.nch + wc0];
-                hi = vci[i * bt.nch + wc1];
+                // entry offsets are multiples of kQuad: work in quads
+                lo = vci[i * bt.nch + wc0] / kQuad;
+                hi = vci[i * bt.nch + wc1] / kQuad;
             }
         }
         uint64_t todo = __ballot(hi > lo);
@@ -870,31 +925,35 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
             }
             if (it[0] < 0)
                 break;
-            uint32_t ev[kItemUnroll];
-            float av[kItemUnroll];
+            uint4 ev[kItemUnroll];
+            float4 av[kItemUnroll];
             bool ok[kItemUnroll];
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
                 ok[u] = lane < n_u[u];
-                const uint32_t *ent_u = ent + lo_u[u];
-                const float *val_u = val + lo_u[u];
+                const uint4 *ent_u = reinterpret_cast<const uint4 *>(ent) + lo_u[u];
+                const float4 *val_u = reinterpret_cast<const float4 *>(val) + lo_u[u];
                 const uint32_t pc = ok[u] ? (uint32_t)lane : 0u;
-                ev[u] = it[u] >= 0 ? ent_u[pc] : 0u;
-                av[u] = it[u] >= 0 ? val_u[pc] : 0.0f;
+                ev[u] = it[u] >= 0 ? ent_u[pc] : make_uint4(0, 0, 0, 0);
+                av[u] = it[u] >= 0 ? val_u[pc] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
             T dv[kItemUnroll];
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
                 const int32_t vm = (int32_t)vb + (it[u] < 0 ? 0 : (it[u] % kItemLanes));
                 const T *du_row = du + (uint32_t)(vm * (int32_t)sa.nc);
-                const T d = du_row[bt_slot(ev[u])] * (T)av[u];
+                // the quad's four entries share one sample (padding: value 0)
+                T d = du_row[bt_slot(ev[u].x)] * (T)av[u].x;
+                d += du_row[bt_slot(ev[u].y)] * (T)av[u].y;
+                d += du_row[bt_slot(ev[u].z)] * (T)av[u].z;
+                d += du_row[bt_slot(ev[u].w)] * (T)av[u].w;
                 dv[u] = ok[u] ? d : T(0);
             }
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
                 if (it[u] < 0)
                     continue;
-                scatter_chunk(ev[u], dv[u], ok[u], lane, acc + (it[u] / kItemLanes) * tile);
+                scatter_chunk(ev[u].x, dv[u], ok[u], lane, acc + (it[u] / kItemLanes) * tile);
             }
         }
     }
@@ -1089,6 +1148,72 @@ __global__ __launch_bounds__(256) void bt_sort_kernel(const int32_t *vc_idx,
         const int64_t lo = ent_off[b] + vc_idx[b * nvc + key];
         const int64_t hi = ent_off[b] + vc_idx[b * nvc + key + 1];
         wave_rank_sort(lo, hi, ent, bval, lane);
+    }
+}
+
+// Padded length of every sorted (view, channel) segment: each run of one
+// sample rounded up to a multiple of kQuad.  One thread per segment.
+__global__ void bt_pad_count_kernel(const int32_t *vc_idx, const int64_t *ent_off,
+                                    int64_t nbatch, int64_t nvc, const uint32_t *ent,
+                                    uint32_t *pcnt)
+{
+    const int64_t nseg = nbatch * (nvc - 1);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nseg;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = t / (nvc - 1), key = t - b * (nvc - 1);
+        const int64_t lo = ent_off[b] + vc_idx[b * nvc + key];
+        const int64_t hi = ent_off[b] + vc_idx[b * nvc + key + 1];
+        uint32_t n = 0, prev = 0xffffffffu, run = 0;
+        for (int64_t p = lo; p < hi; p++) {
+            const uint32_t k = bt_lkey(ent[p]);
+            if (k != prev) {
+                n += (run + kQuad - 1) / kQuad * kQuad;
+                run = 0;
+                prev = k;
+            }
+            run++;
+        }
+        n += (run + kQuad - 1) / kQuad * kQuad;
+        pcnt[b * nvc + key] = n;
+    }
+}
+
+// Copy of the sorted segments with every run padded to a multiple of kQuad
+// (padding: the run's last entry with value 0).  One thread per segment.
+__global__ void bt_pad_fill_kernel(const int32_t *vc_idx, const int64_t *ent_off,
+                                   int64_t nbatch, int64_t nvc, const uint32_t *ent,
+                                   const float *val, const int32_t *pvc,
+                                   const int64_t *pent_off, uint32_t *pent, float *pval)
+{
+    const int64_t nseg = nbatch * (nvc - 1);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nseg;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = t / (nvc - 1), key = t - b * (nvc - 1);
+        const int64_t lo = ent_off[b] + vc_idx[b * nvc + key];
+        const int64_t hi = ent_off[b] + vc_idx[b * nvc + key + 1];
+        int64_t d = pent_off[b] + pvc[b * nvc + key];
+        uint32_t prev = 0xffffffffu, run = 0, last = 0;
+        for (int64_t p = lo; p < hi; p++) {
+            const uint32_t e = ent[p];
+            const uint32_t k = bt_lkey(e);
+            if (k != prev) {
+                for (; run % kQuad; run++, d++) {
+                    pent[d] = last;
+                    pval[d] = 0.0f;
+                }
+                run = 0;
+                prev = k;
+            }
+            pent[d] = e;
+            pval[d] = val[p];
+            d++;
+            run++;
+            last = e;
+        }
+        for (; run % kQuad; run++, d++) {
+            pent[d] = last;
+            pval[d] = 0.0f;
+        }
     }
 }
 
@@ -2216,8 +2341,71 @@ static int build_transposed_batches(ctbb_recon *r, const std::vector<int32_t> &c
                            r->bt_off, nbatch, nvc, r->bt_ent, r->bt_val);
         e = hipGetLastError();
     }
+    // runs of one sample padded to multiples of kQuad entries
+    if (e == hipSuccess) e = hipMemset(d_cnt, 0, nbatch * nvc * 4);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(bt_pad_count_kernel, dim3(16384), dim3(256), 0, 0, r->bt_vc,
+                           r->bt_off, nbatch, nvc, r->bt_ent, d_cnt);
+        e = hipGetLastError();
+    }
+    int32_t *pvc = nullptr;
+    int64_t *poff = nullptr;
+    uint32_t *pent = nullptr;
+    float *pval = nullptr;
+    auto free_pad = [&]() {
+        if (pvc) (void)hipFree(pvc);
+        if (poff) (void)hipFree(poff);
+        if (pent) (void)hipFree(pent);
+        if (pval) (void)hipFree(pval);
+    };
+    if (e == hipSuccess && ((rc = dev_alloc_n(&pvc, nbatch * nvc, "segment index")) ||
+                            (rc = dev_alloc_n(&poff, nbatch, "batch offsets")))) {
+        free_pad();
+        cleanup();
+        return rc;
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(bt_scan_kernel, dim3((unsigned)nbatch), dim3(1024), 0, 0,
+                           d_cnt, pvc, d_tot, nvc);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(tot.data(), d_tot, nbatch * 8, hipMemcpyDeviceToHost);
+    run = 0;
+    for (int64_t b = 0; b < nbatch; b++) {
+        off[b] = run;
+        run += tot[b];
+    }
+    if (e == hipSuccess && run >= ((int64_t)1 << 31) * kQuad)
+        e = hipErrorInvalidValue;
+    if (e == hipSuccess &&
+        ((rc = dev_alloc_n(&pent, std::max<int64_t>(1, run), "transposed entries")) ||
+         (rc = dev_alloc_n(&pval, std::max<int64_t>(1, run), "transposed values")))) {
+        free_pad();
+        cleanup();
+        return rc;
+    }
+    if (e == hipSuccess) e = hipMemcpy(poff, off.data(), nbatch * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(bt_pad_fill_kernel, dim3(16384), dim3(256), 0, 0, r->bt_vc,
+                           r->bt_off, nbatch, nvc, r->bt_ent, r->bt_val, pvc, poff, pent,
+                           pval);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess)
         e = hipDeviceSynchronize();
+    if (e == hipSuccess) {
+        (void)hipFree(r->bt_vc);
+        (void)hipFree(r->bt_off);
+        (void)hipFree(r->bt_ent);
+        (void)hipFree(r->bt_val);
+        r->bt_vc = pvc;
+        r->bt_off = poff;
+        r->bt_ent = pent;
+        r->bt_val = pval;
+    } else {
+        free_pad();
+    }
     cleanup();
     if (e != hipSuccess)
         return fail(CTBB_E_HIP, std::string("transposed batches: ") + hipGetErrorString(e));
@@ -2298,15 +2486,31 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
     std::vector<int32_t> cols;
     std::vector<int64_t> bptr(1, 0);
     const int64_t ntx = (g->nx + tile - 1) / tile, nty = (g->ny + tile - 1) / tile;
+    // inside a batch the columns (its slots) follow the Z-order curve over
+    // the tile grid: the columns along any ray through the image then have
+    // nearby slots, so the scatter's reads of their updates share cache
+    // lines (a batch is one Jacobi step: the order inside it does not change
+    // the result)
+    std::vector<std::pair<uint64_t, int64_t>> zt;
+    zt.reserve(ntx * nty);
+    for (int64_t ty = 0; ty < nty; ty++)
+        for (int64_t tx = 0; tx < ntx; tx++) {
+            uint64_t z = 0;
+            for (int bit = 0; bit < 32; bit++)
+                z |= (((uint64_t)tx >> bit) & 1u) << (2 * bit) |
+                     (((uint64_t)ty >> bit) & 1u) << (2 * bit + 1);
+            zt.emplace_back(z, ty * ntx + tx);
+        }
+    std::sort(zt.begin(), zt.end());
     for (int64_t ly = 0; ly < tile; ly++)
         for (int64_t lx = 0; lx < tile; lx++) {
             const size_t before = cols.size();
-            for (int64_t ty = 0; ty < nty; ty++)
-                for (int64_t tx = 0; tx < ntx; tx++) {
-                    const int64_t i = tx * tile + lx, j = ty * tile + ly;
-                    if (i < g->nx && j < g->ny && in_fov(i, j))
-                        cols.push_back((int32_t)(i + g->nx * j));
-                }
+            for (const auto &t : zt) {
+                const int64_t ty = t.second / ntx, tx = t.second - ty * ntx;
+                const int64_t i = tx * tile + lx, j = ty * tile + ly;
+                if (i < g->nx && j < g->ny && in_fov(i, j))
+                    cols.push_back((int32_t)(i + g->nx * j));
+            }
             if (cols.size() > before)
                 bptr.push_back((int64_t)cols.size());
         }
