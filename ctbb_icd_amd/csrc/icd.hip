@@ -477,16 +477,19 @@ struct StepArgs {
 // per chunk, in entry order: deterministic).  Per member, each lane adds its
 // rank's kViewBlock accumulators in view order and writes the partial of
 // (member, block, slot); phase A2 adds the partials over the blocks in a
-// fixed order.  Workgroups are dispatched in block order, so the chunks of
-// one block -- which share the error-sinogram lines of a member's band --
-// run together on one XCD (logical workgroups go to the XCDs in contiguous
-// ranges).  FP16 matrices: values are summed unscaled and the per-column
+// fixed order.  Workgroups are dispatched by view block, the blocks in
+// residue classes of P = the view distance of consecutive group members in
+// blocks (ib, ib + P, ib + 2P, ...): the chunks of one block share the
+// error-sinogram lines of each member's band, and member m + 1 of block ib
+// reads the same data views as member m of block ib + P, so those lines are
+// fetched into L2 once per class instead of once per member (logical
+// workgroups go to the XCDs in contiguous ranges).  FP16 matrices: values are summed unscaled and the per-column
 // scale is applied in phase A2 (it factors out of both sums).
 #ifndef CTBB_VIEW_BLOCK
 #define CTBB_VIEW_BLOCK 4
 #endif
 #ifndef CTBB_GATHER_UNROLL
-#define CTBB_GATHER_UNROLL 2
+#define CTBB_GATHER_UNROLL 4
 #endif
 #ifndef CTBB_GATHER_MEMBERS
 #define CTBB_GATHER_MEMBERS 4
@@ -518,6 +521,8 @@ struct GatherArgs {
     const void *ssum;         // T: batch surrogate, sum of the step's values
                               //   per data sample (null: sum w a^2)
     int64_t nsc;              // workgroups per view block
+    int64_t pchain, rows;     // dispatch: view blocks in residue classes of
+                              // pchain (rows blocks per class)
     int64_t nsm, readings;
 };
 
@@ -550,13 +555,17 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
     // bases below are then scalar too
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int64_t nwg = g.nib * g.nsc;
+    const int64_t nwg = g.pchain * g.rows * g.nsc;
     const int64_t per_xcd = (nwg + kXcd - 1) / kXcd;
     const int64_t L = (int64_t)(blockIdx.x % kXcd) * per_xcd + blockIdx.x / kXcd;
     if (L >= nwg)
         return;
-    const int64_t ib = L / g.nsc;
-    const int64_t w = (L - ib * g.nsc) * kWaves + wave;
+    const int64_t lb = L / g.nsc;
+    const int64_t cls = lb / g.rows;
+    const int64_t ib = cls + (lb - cls * g.rows) * g.pchain;
+    if (ib >= g.nib)
+        return;
+    const int64_t w = (L - lb * g.nsc) * kWaves + wave;
     if (w >= g.nw)
         return;
     const int64_t *ptr = g.ptr + g.wbase + ib * g.nw;
@@ -3117,7 +3126,15 @@ static int recon_iterate_t(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                 ga.wbase = r->h_wbase[b];
                 ga.nw = (nc + 63) / 64;
                 ga.nsc = (ga.nw + kWaves - 1) / kWaves;
-                const int64_t nwg = r->nib * ga.nsc;
+                // view blocks in residue classes of the view stride between
+                // consecutive group members (in blocks): blocks ib and
+                // ib + pchain -- dispatched together -- read the same data
+                // views of the error sinogram for members m + 1 and m
+                ga.pchain = std::min<int64_t>(
+                    std::max<int64_t>(1, (r->view_stride[j] + kViewBlock / 2) / kViewBlock),
+                    r->nib);
+                ga.rows = (r->nib + ga.pchain - 1) / ga.pchain;
+                const int64_t nwg = ga.pchain * ga.rows * ga.nsc;
                 const int64_t ntask = kXcd * ((nwg + kXcd - 1) / kXcd);
                 int64_t vlo = INT64_MAX, vhi = INT64_MIN;
                 for (int64_t k = j; k < r->g.nz; k += r->spacing) {
