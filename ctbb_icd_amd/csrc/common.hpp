@@ -125,10 +125,9 @@ struct ctbb_recon {
     // view-blocked batch copy of the matrix (gather): per batch b and view
     // block ib (kViewBlock views) the columns ranked by detector position,
     // in chunks of 64 ranks; a chunk's entries ordered (view in block, rank)
-    int32_t *gb_idx = nullptr;    // [fov entries] ray index (as in the SSM)
+    uint32_t *gb_ent = nullptr;   // [fov entries] sample in view | key << 17
     float *gb_val = nullptr;      // [fov entries] FP32 value (FP32 matrices)
     __half *gb_val16 = nullptr;   // [fov entries] FP16 value (FP16 matrices)
-    uint8_t *gb_key = nullptr;    // [fov entries] view in block << 6 | rank % 64
     uint16_t *gb_perm = nullptr;  // [nib*ncols_fov] column slot of (b, ib, rank)
     int64_t *gb_ptr = nullptr;    // [chunks+1] start of (b, ib, 64-rank chunk)
     std::vector<int64_t> h_wbase; // [nbatch+1] first chunk of each batch

@@ -505,11 +505,16 @@ constexpr int kGatherMembers = CTBB_GATHER_MEMBERS;
 constexpr int kGatherMembersFirst = CTBB_GATHER_MEMBERS_FIRST;
 static_assert(kGatherMembers >= 1 && kGatherMembersFirst >= 1, "members per pass");
 
+// Packed view-blocked entry: the sample in its view in the low kSampleBits
+// bits, the key (view in block << 6 | rank % 64) above them.
+constexpr uint32_t kSampleBits = 17, kSampleMask = (1u << kSampleBits) - 1;
+
 struct GatherArgs {
-    const int32_t *idx;       // view-blocked copy (see above)
+    const uint32_t *ent;      // view-blocked copy (see above): sample in view
+                              //   | (view in block << 6 | rank % 64) << kSampleBits
     const float *val;
     const __half *val16;
-    const uint8_t *key;       // view in block << 6 | rank % 64
+    int64_t plane;            // samples per view (n_rows * n_channels)
     const uint16_t *perm;     // column slot of each (b, ib, rank)
     const int64_t *ptr;       // starts of the (b, ib, chunk) entry ranges
     int64_t base;             // nib * (first FOV index of the batch)
@@ -606,11 +611,14 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                 accm[m][wave * kAcc + q * 64 + lane] = T(0);
         // the chunk's entries from scalar bases with 32-bit lane offsets,
         // clamped into the chunk (a chunk is non-empty inside the loop)
-        const int32_t *idx_c = g.idx + lo;
+        const uint32_t *ent_c = g.ent + lo;
         const float *val_c = FP16 ? nullptr : g.val + lo;
         const __half *val16_c = FP16 ? g.val16 + lo : nullptr;
-        const uint8_t *key_c = g.key + lo;
         const uint32_t n = (uint32_t)(hi - lo);
+        // ray index = sample in view + (first view of the block + view in
+        // block) * plane; below 2^31 (checked at set-up)
+        const int32_t ib_base = (int32_t)(ib * kViewBlock * g.plane);
+        const int32_t plane = (int32_t)g.plane;
         for (uint32_t p0 = 0; p0 < n; p0 += 64 * kGatherUnroll) {
             int32_t iv[kGatherUnroll], kv[kGatherUnroll];
             float av[kGatherUnroll];
@@ -619,9 +627,10 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                 const uint32_t p = p0 + u * 64 + lane;
                 const bool ok = p < n;
                 const uint32_t pc = ok ? p : 0u;
-                const int32_t i0 = idx_c[pc];
+                const uint32_t e0 = ent_c[pc];
                 const float a0 = FP16 ? __half2float(val16_c[pc]) : val_c[pc];
-                const int32_t k0 = (int32_t)key_c[pc];
+                const int32_t k0 = (int32_t)(e0 >> kSampleBits);
+                const int32_t i0 = (int32_t)(e0 & kSampleMask) + ib_base + (k0 >> 6) * plane;
                 iv[u] = ok ? i0 : 0;
                 av[u] = ok ? a0 : 0.0f;
                 kv[u] = ok ? k0 : -1;
@@ -883,12 +892,14 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
     for (int i = tid; i < kScatterViews * tile; i += kSweepBlock)
         acc[i] = T(0);
     __syncthreads();
-    // items: lane l of a round covers view u = l / kItemLanes and member
-    // vb + l % kItemLanes
+    // items: lane l of a round covers member vb + l / kScatterViews and view
+    // u = l % kScatterViews, so one member's items come one after another
+    // (the columns of neighbouring views are nearly the same: their du reads
+    // hit the lines the previous item brought in)
     constexpr int kItemLanes = 64 / kScatterViews;
     for (int64_t vb = 0; vb < sa.ng; vb += kItemLanes) {
-        const int u_l = lane / kItemLanes;
-        const int64_t v_l = vb + (lane % kItemLanes);
+        const int u_l = lane % kScatterViews;
+        const int64_t v_l = vb + lane / kScatterViews;
         int32_t lo = 0, hi = 0;
         if (u_l < nview && v_l < sa.ng && wc1 > wc0) {
             const int64_t i = view0 + u_l - bt.si_half[sa.j + a.spacing * v_l];
@@ -949,7 +960,7 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
             T dv[kItemUnroll];
 #pragma unroll
             for (int u = 0; u < kItemUnroll; u++) {
-                const int32_t vm = (int32_t)vb + (it[u] < 0 ? 0 : (it[u] % kItemLanes));
+                const int32_t vm = (int32_t)vb + (it[u] < 0 ? 0 : (it[u] / kScatterViews));
                 const T *du_row = du + (uint32_t)(vm * (int32_t)sa.nc);
                 // the quad's four entries share one sample (padding: value 0)
                 T d = du_row[bt_slot(ev[u].x)] * (T)av[u].x;
@@ -962,7 +973,8 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
             for (int u = 0; u < kItemUnroll; u++) {
                 if (it[u] < 0)
                     continue;
-                scatter_chunk(ev[u].x, dv[u], ok[u], lane, acc + (it[u] / kItemLanes) * tile);
+                scatter_chunk(ev[u].x, dv[u], ok[u], lane,
+                              acc + (it[u] % kScatterViews) * tile);
             }
         }
     }
@@ -1990,10 +2002,9 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
     for (hipEvent_t ev : r->ev_step)
         if (ev) (void)hipEventDestroy(ev);
     if (r->part2) (void)hipFree(r->part2);
-    if (r->gb_idx) (void)hipFree(r->gb_idx);
+    if (r->gb_ent) (void)hipFree(r->gb_ent);
     if (r->gb_val) (void)hipFree(r->gb_val);
     if (r->gb_val16) (void)hipFree(r->gb_val16);
-    if (r->gb_key) (void)hipFree(r->gb_key);
     if (r->gb_perm) (void)hipFree(r->gb_perm);
     if (r->gb_ptr) (void)hipFree(r->gb_ptr);
     if (r->stream) (void)hipStreamDestroy(r->stream);
@@ -2099,8 +2110,8 @@ __global__ __launch_bounds__(256) void gb_chunk_kernel(
     const int64_t *col_ptr, const int32_t *idx, const float *val,
     const __half *val16, const int32_t *view_off, const int32_t *cols,
     const int64_t *bptr, const int64_t *wbase, const uint16_t *perm, int64_t nsm,
-    int64_t nib, uint32_t *wcnt, const int64_t *gptr, int32_t *gidx, float *gval,
-    __half *gval16, uint8_t *gkey)
+    int64_t nib, uint32_t *wcnt, const int64_t *gptr, uint32_t *gent, float *gval,
+    __half *gval16, int64_t plane)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t b = blockIdx.x / nib, ib = blockIdx.x - b * nib;
@@ -2121,14 +2132,13 @@ __global__ __launch_bounds__(256) void gb_chunk_kernel(
             if (FILL && n > 0) {
                 const int64_t src = col_ptr[col] + view_off[col * nsm + v];
                 const int64_t dst = seg + inc - n;
-                const uint8_t kb = (uint8_t)(q << 6 | lane);
+                const uint32_t kb = (uint32_t)(q << 6 | lane) << kSampleBits;
                 for (int64_t e = 0; e < n; e++) {
-                    gidx[dst + e] = idx[src + e];
+                    gent[dst + e] = (uint32_t)(idx[src + e] - v * plane) | kb;
                     if (gval16)
                         gval16[dst + e] = val16[src + e];
                     else
                         gval[dst + e] = val[src + e];
-                    gkey[dst + e] = kb;
                 }
             }
             seg += tot;
@@ -2145,7 +2155,10 @@ static int build_gather_blocks(ctbb_recon *r, const std::vector<int64_t> &bptr)
     const int64_t nsm = g->n_views_sm, nib = (nsm + kViewBlock - 1) / kViewBlock;
     const int64_t ncols = r->ncols_fov, n = nib * ncols;
     const bool fp16 = m->precision == CTBB_SSM_FP16;
-    static_assert(kViewBlock <= 4, "the entry byte holds the view in block in 2 bits");
+    static_assert(kViewBlock <= 4, "the entry key holds the view in block in 2 bits");
+    if (g->n_rows * g->n_channels > (int64_t)kSampleMask + 1)
+        return fail(CTBB_E_UNSUPPORTED, "view-blocked batches need n_rows * n_channels "
+                                        "<= 2^17");
     r->nib = nib;
     std::vector<int64_t> base(ncols);
     std::vector<int32_t> nc(ncols), slot(ncols);
@@ -2192,8 +2205,7 @@ static int build_gather_blocks(ctbb_recon *r, const std::vector<int64_t> &bptr)
         (rc = dev_alloc_n(&sums, std::max<int64_t>(1, nscan), "chunk sums")) ||
         (rc = dev_alloc_n(&r->gb_perm, std::max<int64_t>(1, n), "rank slots")) ||
         (rc = dev_alloc_n(&r->gb_ptr, nchunks + 1, "chunk pointers")) ||
-        (rc = dev_alloc_n(&r->gb_idx, ne, "blocked indices")) ||
-        (rc = dev_alloc_n(&r->gb_key, ne, "blocked keys")) ||
+        (rc = dev_alloc_n(&r->gb_ent, ne, "blocked entries")) ||
         (rc = fp16 ? dev_alloc_n(&r->gb_val16, ne, "blocked values")
                    : dev_alloc_n(&r->gb_val, ne, "blocked values"))) {
         cleanup();
@@ -2216,7 +2228,7 @@ static int build_gather_blocks(ctbb_recon *r, const std::vector<int64_t> &bptr)
         hipLaunchKernelGGL(gb_chunk_kernel<false>, dim3((unsigned)nblk), dim3(256), 0, 0,
                            m->col_ptr, m->idx, m->val, m->val16, m->view_off, r->cols,
                            r->batch_ptr, d_wbase, r->gb_perm, nsm, nib, wcnt, nullptr,
-                           nullptr, nullptr, nullptr, nullptr);
+                           nullptr, nullptr, nullptr, g->n_rows * g->n_channels);
         hipLaunchKernelGGL(chunk_sum_kernel, dim3((unsigned)nscan), dim3(256), 0, 0, wcnt,
                            nchunks, sums);
         e = hipGetLastError();
@@ -2245,7 +2257,7 @@ static int build_gather_blocks(ctbb_recon *r, const std::vector<int64_t> &bptr)
         hipLaunchKernelGGL(gb_chunk_kernel<true>, dim3((unsigned)nblk), dim3(256), 0, 0,
                            m->col_ptr, m->idx, m->val, m->val16, m->view_off, r->cols,
                            r->batch_ptr, d_wbase, r->gb_perm, nsm, nib, nullptr, r->gb_ptr,
-                           r->gb_idx, r->gb_val, r->gb_val16, r->gb_key);
+                           r->gb_ent, r->gb_val, r->gb_val16, g->n_rows * g->n_channels);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
@@ -3064,10 +3076,10 @@ static int recon_iterate_t(ctbb_recon *r, int n, ctbb_icd_stats *stats)
     a.th2 = r->th2;
     const bool fp16 = r->gb_val16 != nullptr;
     GatherArgs ga;
-    ga.idx = r->gb_idx;
+    ga.ent = r->gb_ent;
+    ga.plane = r->g.n_rows * r->g.n_channels;
     ga.val = r->gb_val;
     ga.val16 = r->gb_val16;
-    ga.key = r->gb_key;
     ga.perm = r->gb_perm;
     ga.ptr = r->gb_ptr;
     ga.part1 = r->part1;
