@@ -121,7 +121,6 @@ struct ctbb_recon {
     // sample r; zero between steps (first sweep only)
     void *ssum = nullptr;         // [data_size] or null (plain curvature)
     void *du_m1 = nullptr;        // [(max_batch+8)*nz] all -1: scatter S += a
-    void *du_p1 = nullptr;        // [(max_batch+8)*nz] all +1: scatter S -= a
     // view-blocked batch copy of the matrix (gather): per batch b and view
     // block ib (kViewBlock views) the columns ranked by detector position,
     // in chunks of 64 ranks; a chunk's entries ordered (view in block, rank)

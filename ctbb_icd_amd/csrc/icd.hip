@@ -497,7 +497,7 @@ struct StepArgs {
 constexpr int kViewBlock = CTBB_VIEW_BLOCK;
 constexpr int kGatherUnroll = CTBB_GATHER_UNROLL;
 #ifndef CTBB_GATHER_MEMBERS_FIRST
-#define CTBB_GATHER_MEMBERS_FIRST 2
+#define CTBB_GATHER_MEMBERS_FIRST 4
 #endif
 // Members of the slice group a wavefront takes per pass over its entries in
 // the later sweeps and in the first sweep (which also sums theta2)
@@ -1238,6 +1238,15 @@ __global__ void bt_pad_fill_kernel(const int32_t *vc_idx, const int64_t *ent_off
     }
 }
 
+// p[0, n) = 0 (a plain store kernel, so that a caller may capture the sweep)
+template <typename T>
+__global__ __launch_bounds__(256) void zero_kernel(T *p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        p[i] = T(0);
+}
+
 // ---------------------------------------------------------------------------
 // Reductions, resync and cost.
 
@@ -1502,13 +1511,26 @@ __global__ __launch_bounds__(kSweepBlock) void project_gather_kernel(
     const int64_t *si_half, const int32_t *k_lo, const int32_t *k_hi,
     int64_t nsm, int64_t nch, int64_t nr, int64_t nxy, int64_t ncg,
     double *yhat, const float *y, const float *w, E *e, E *ew,
-    double *partials)
+    double *partials, int64_t readings, int64_t pstride, int64_t rows)
 {
     extern __shared__ double pacc[];
     __shared__ double sh[kWaves];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t view = blockIdx.x / ncg;
-    const int64_t c0 = (blockIdx.x % ncg) * kScatterCh;
+    // data views in residue classes of pstride, the view distance of
+    // neighbouring slices: views v and v + pstride read the same SSM views
+    // for slices k and k + 1, so consecutive workgroups share the entries in
+    // L2; logical workgroups go to the XCDs in contiguous ranges
+    const int64_t nwg = pstride * rows * ncg;
+    const int64_t per_xcd = (nwg + kXcd - 1) / kXcd;
+    const int64_t L = (int64_t)(blockIdx.x % kXcd) * per_xcd + blockIdx.x / kXcd;
+    if (L >= nwg)
+        return;
+    const int64_t ql = L / ncg, cg = L - ql * ncg;
+    const int64_t cls = ql / rows;
+    const int64_t view = cls + (ql - cls * rows) * pstride;
+    if (view >= readings)
+        return;
+    const int64_t c0 = cg * kScatterCh;
     const int64_t c1 = min(c0 + kScatterCh, nch);
     const int nsamp = (int)((c1 - c0) * nr);
     for (int i = tid; i < nsamp; i += kSweepBlock)
@@ -1571,7 +1593,7 @@ __global__ __launch_bounds__(kSweepBlock) void project_gather_kernel(
     if (MODE == 1) {
         const double s = block_sum_f64(part, sh);
         if (tid == 0)
-            partials[blockIdx.x] = s;
+            partials[view * ncg + cg] = s;
     }
 }
 
@@ -1793,10 +1815,19 @@ int launch_project(const ctbb_ssm *m, const X *x, const int64_t *d_si_half,
     const ctbb_geom *g = &m->g;
     const int64_t ncg = (g->n_channels + kScatterCh - 1) / kScatterCh;
     const size_t lds = sizeof(double) * kScatterCh * g->n_rows;
-    hipLaunchKernelGGL((project_gather_kernel<X, E, MODE>), dim3((unsigned)project_grid(g)),
+    // view distance of neighbouring slices (a dispatch order only)
+    int64_t pstride = 1;
+    if (g->tube_z_increment != 0.0)
+        pstride = (int64_t)llround(fabs(g->voxel_size_z / g->tube_z_increment));
+    pstride = std::min<int64_t>(std::max<int64_t>(pstride, 1), g->readings);
+    const int64_t rows = (g->readings + pstride - 1) / pstride;
+    const int64_t nwg = pstride * rows * ncg;
+    const int64_t grid = kXcd * ((nwg + kXcd - 1) / kXcd);
+    hipLaunchKernelGGL((project_gather_kernel<X, E, MODE>), dim3((unsigned)grid),
                        dim3(kSweepBlock), lds, st, m->rm_vc, m->rm_ent, m->rm_val, x,
                        d_si_half, d_klo, d_khi, g->n_views_sm, g->n_channels, g->n_rows,
-                       g->nx * g->ny, ncg, yhat, y, w, e, ew, partials);
+                       g->nx * g->ny, ncg, yhat, y, w, e, ew, partials, g->readings,
+                       pstride, rows);
     CTBB_HIP(hipGetLastError());
     return CTBB_OK;
 }
@@ -1996,7 +2027,6 @@ extern "C" int ctbb_recon_free(ctbb_recon *r)
     if (r->ssum) (void)hipFree(r->ssum);
     if (r->du_m1) (void)hipFree(r->du_m1);
     if (r->shear) (void)hipFree(r->shear);
-    if (r->du_p1) (void)hipFree(r->du_p1);
     for (hipEvent_t ev : r->ev_sweep)
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : r->ev_step)
@@ -2650,25 +2680,22 @@ extern "C" int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
             return bail(rc);
         r->th2_valid = false;
         // batch surrogate sample sums (one step at a time, first sweep only):
-        // zeroed once here, filled and emptied by the +-1 scatters of a step;
+        // zeroed once here, filled by the -1 scatter of a step and set back
+        // to zero over the step's data views after it;
         // single-column batches keep the plain curvature (their conflict-free
         // slice groups share no sample, where S would equal a)
         const bool single = tile >= std::max(g->nx, g->ny);
         if (!opts->plain_curvature && !single) {
             const int64_t nd = (max_batch + 8) * g->nz;
             if ((rc = dev_alloc(&r->ssum, r->data_size * r->sb, "surrogate sample sums")) ||
-                (rc = dev_alloc(&r->du_m1, nd * r->sb, "surrogate scatter weights")) ||
-                (rc = dev_alloc(&r->du_p1, nd * r->sb, "surrogate scatter weights")))
+                (rc = dev_alloc(&r->du_m1, nd * r->sb, "surrogate scatter weights")))
                 return bail(rc);
-            std::vector<float> m1(nd, -1.0f), p1(nd, 1.0f);
-            std::vector<double> m1d(nd, -1.0), p1d(nd, 1.0);
+            std::vector<float> m1(nd, -1.0f);
+            std::vector<double> m1d(nd, -1.0);
             const void *hm = r->fp64 ? (const void *)m1d.data() : (const void *)m1.data();
-            const void *hp = r->fp64 ? (const void *)p1d.data() : (const void *)p1.data();
             hipError_t he2 = hipMemset(r->ssum, 0, r->data_size * r->sb);
             if (he2 == hipSuccess)
                 he2 = hipMemcpy(r->du_m1, hm, nd * r->sb, hipMemcpyHostToDevice);
-            if (he2 == hipSuccess)
-                he2 = hipMemcpy(r->du_p1, hp, nd * r->sb, hipMemcpyHostToDevice);
             if (he2 != hipSuccess)
                 return bail(fail(CTBB_E_HIP, std::string("surrogate set-up: ") +
                                                  hipGetErrorString(he2)));
@@ -3201,12 +3228,12 @@ static int recon_iterate_t(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                     CTBB_HIP(hipEventRecord(ev[4 * used + 3], r->stream));
                     used++;
                 }
-                // ... and S -= a in the same order: exactly zero again
+                // ... and S back to zero: the step touched data views [vlo, vhi) only
                 if (first && r->ssum && scatter_grid > 0) {
-                    as.du = r->du_p1;
-                    hipLaunchKernelGGL(icd_scatter_kernel<T>, dim3((unsigned)scatter_grid),
-                                       dim3(kSweepBlock), scatter_lds, r->stream, as, sa, bt,
-                                       b, so);
+                    const int64_t plane = r->g.n_channels * r->g.n_rows;
+                    hipLaunchKernelGGL(zero_kernel<T>, dim3(8192), dim3(256), 0, r->stream,
+                                       static_cast<T *>(r->ssum) + vlo * plane,
+                                       (vhi - vlo) * plane);
                 }
                 launches += 3;
                 step++;

@@ -453,43 +453,60 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad_tiled(ConvIn in, const uint
     }
 }
 
-// g[i] = sum over chunks of part[chunk][i], in chunk order.
+// g[i] = sum over chunks of part[chunk][i]: eight running sums (chunks c,
+// c + 8, c + 16, ...; eight loads in flight) added in a fixed order.
 __global__ void sum_chunks_kernel(const float *part, int64_t nchunk, int64_t n, float *g)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        float s = 0.0f;
-        for (int64_t c = 0; c < nchunk; c++)
-            s += part[c * n + i];
-        g[i] = s;
+        float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        int64_t c = 0;
+        for (; c + 8 <= nchunk; c += 8) {
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                s[k] += part[(c + k) * n + i];
+        }
+        for (int k = 0; c < nchunk; c++, k++)
+            s[k] += part[c * n + i];
+        g[i] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
     }
 }
 
 // Bias gradient partials: part[chunk][co] = sum over the chunk's pixels of
-// dY[p][co].  Threads t of a workgroup take channel t % cout and pixels
-// chunk_start + t / cout, + 256 / cout, ... (coalesced rows of dY); the
-// 256 / cout partial sums of a channel are added in LDS in a fixed order.
-// Needs cout to divide 256.
-constexpr int64_t kBiasChunk = 4096;
+// dY[p][co].  Thread t takes the 8 channels 8 (t % g8) .. + 7 (one 16-byte
+// load of a pixel row; g8 = cout / 8) of pixels chunk_start + t / g8,
+// + 256 / g8, ...; the partial sums of a channel are added in LDS in a
+// fixed order.  Needs cout a multiple of 8 dividing 2048.
+constexpr int64_t kBiasChunk = 1024;
 
 __global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t *dy, int64_t P,
                                                         int cout, float *part)
 {
-    __shared__ float sh[256];
+    __shared__ float sh[256 * 8];
     const int t = threadIdx.x;
-    const int c = t % cout, lanes = 256 / cout, l = t / cout;
+    const int g8 = cout / 8, lanes = 256 / g8;
+    const int cg = t % g8, l = t / g8;
     const int64_t p0 = (int64_t)blockIdx.x * kBiasChunk;
     const int64_t p1 = min(p0 + kBiasChunk, P);
-    float s = 0.0f;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (l < lanes)
-        for (int64_t p = p0 + l; p < p1; p += lanes)
-            s += bf2f(dy[p * cout + c]);
-    sh[t] = s;
+        for (int64_t p = p0 + l; p < p1; p += lanes) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(dy + p * cout + 8 * cg);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                s[2 * k] += __uint_as_float(w[k] << 16);
+                s[2 * k + 1] += __uint_as_float(w[k] & 0xffff0000u);
+            }
+        }
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        sh[t * 8 + k] = s[k];
     __syncthreads();
-    if (l == 0) {
+    for (int c = t; c < cout; c += 256) {
         float tot = 0.0f;
         for (int i = 0; i < lanes; i++)
-            tot += sh[i * cout + c];
+            tot += sh[(i * g8 + c / 8) * 8 + c % 8];
         part[(int64_t)blockIdx.x * cout + c] = tot;
     }
 }

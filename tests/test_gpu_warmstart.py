@@ -112,3 +112,46 @@ def test_shipped_network_gains_over_plain_at_equal_time(ct, make_scan):
           f"{(c_plain - c_warm) / c_warm:.2e})", flush=True)
     assert n_plain >= 1 + n_after
     assert c_warm < c_plain
+
+
+def test_config3_pipeline_checks_the_start_and_reaches_target(ct, make_scan):
+    """The pipeline at the reduced-size learned-start scan (config3: the
+    clinical geometry cut to 256 x 256 x 32, 1.4 mm pixels): plain sweeps
+    lower the cost; a network whose start is far off (+50 mu_water) is
+    rejected by the cost check and the run continues from the plain image,
+    reaching the target; the untrained network (the identity plus the
+    de-rotate / rotate resampling) is judged by the same check and its run
+    reaches the target too."""
+    from ctbb_icd_amd import warmstart as ws
+    ct.init(0)
+    s = make_scan("config3")
+    g = s.geom
+    assert (g.nx, g.ny, g.nz) == (256, 256, 32)
+    ssm = ct.SSM.build(g)
+    raw, w = s.synthesize(use_gpu=True)
+    r = ct.Recon(ssm, raw, w, s.slice_index, s.opts())
+    costs = []
+    for _ in range(8):
+        r.iterate(1)
+        costs.append(r.cost())
+    r.free()
+    assert all(b < a for a, b in zip(costs, costs[1:])), costs
+    target = costs[-1] * (1 + 1e-3)
+    bad = ct.UNet(3, 8, 8, g.ny, g.nx, seed=2)
+    p = bad.params()
+    p[-1] = 50.0
+    bad.set_params(p)
+    run = ws.warm_run(s, ssm, raw, w, bad, 1, target, 40)
+    assert run.accepted is False
+    assert run.costs[0] == pytest.approx(costs[0], rel=1e-9)
+    assert run.reached
+    ident = ct.UNet(3, 8, 8, g.ny, g.nx, seed=2)
+    run2 = ws.warm_run(s, ssm, raw, w, ident, 1, target, 40)
+    assert run2.accepted in (True, False)
+    assert run2.reached
+    print(f"[config3] plain costs {costs[0]:.6e} .. {costs[-1]:.6e}; bad start rejected, "
+          f"{run.iterations} its; identity start "
+          f"{'accepted' if run2.accepted else 'rejected'}, {run2.iterations} its", flush=True)
+    bad.free()
+    ident.free()
+    ssm.free()
