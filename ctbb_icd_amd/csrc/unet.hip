@@ -566,40 +566,77 @@ __global__ void maxpool_bwd_kernel(const uint16_t *in, const uint16_t *dout, uin
 // dY[i] = 0 where the ReLU output Y[i] is 0.
 __global__ void relu_mask_kernel(const uint16_t *y, uint16_t *dy, int64_t n)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        if (!(bf2f(y[i]) > 0.0f))
-            dy[i] = 0;
+    // eight values per thread (16-byte loads; n is a multiple of 8): dy kept
+    // where y > 0, i.e. the BF16 bits of y in [1, 0x7f80] (NaN and <= 0 fail)
+    const uint4 *y8 = reinterpret_cast<const uint4 *>(y);
+    uint4 *d8 = reinterpret_cast<uint4 *>(dy);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n / 8;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint4 a = y8[i];
+        uint4 d = d8[i];
+        uint32_t *dw = reinterpret_cast<uint32_t *>(&d);
+        const uint32_t aw[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t lo = aw[k] & 0xffffu, hi = aw[k] >> 16;
+            const uint32_t keep = ((lo - 1u) < 0x7f80u ? 0x0000ffffu : 0u) |
+                                  ((hi - 1u) < 0x7f80u ? 0xffff0000u : 0u);
+            dw[k] &= keep;
+        }
+        d8[i] = d;
+    }
 }
 
 // Gradient of a concatenated input: the first ca channels are added to dA
 // (in FP32, rounded to BF16); the cb upsampled channels are summed over their
-// 2x2 source block into dB (half resolution), in a fixed order.
+// 2x2 source block into dB (half resolution), in a fixed order.  Eight
+// channels per thread (16-byte loads; ca and cb are multiples of 8).
 __global__ void split_concat_grad_kernel(const uint16_t *dx, int N, int H, int W, int ca,
                                          int cb, uint16_t *da, uint16_t *db)
 {
-    const int64_t na = (int64_t)N * H * W * ca;
+    const int a8 = ca / 8, b8 = cb / 8;
+    const int64_t na = (int64_t)N * H * W * a8;
     const int H2 = H >> 1, W2 = W >> 1;
-    const int64_t nb = (int64_t)N * H2 * W2 * cb;
+    const int64_t nb = (int64_t)N * H2 * W2 * b8;
     const int C = ca + cb;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb;
          i += (int64_t)gridDim.x * blockDim.x) {
         if (i < na) {
-            const int c = (int)(i % ca);
-            const int64_t p = i / ca;
-            da[i] = f2bf(bf2f(da[i]) + bf2f(dx[p * C + c]));
+            const int c8 = (int)(i % a8);
+            const int64_t p = i / a8;
+            uint4 *pa = reinterpret_cast<uint4 *>(da + p * ca + 8 * c8);
+            const uint4 x = *reinterpret_cast<const uint4 *>(dx + p * C + 8 * c8);
+            uint4 v = *pa;
+            const uint16_t *xv = reinterpret_cast<const uint16_t *>(&x);
+            uint16_t *vv = reinterpret_cast<uint16_t *>(&v);
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                vv[k] = f2bf(bf2f(vv[k]) + bf2f(xv[k]));
+            *pa = v;
         } else {
             const int64_t j = i - na;
-            const int c = (int)(j % cb);
-            int64_t t = j / cb;
+            const int c8 = (int)(j % b8);
+            int64_t t = j / b8;
             const int x = (int)(t % W2);
             t /= W2;
             const int y = (int)(t % H2);
             const int n = (int)(t / H2);
-            float s = 0.0f;
-            for (int k = 0; k < 4; k++)
-                s += bf2f(dx[(((int64_t)n * H + 2 * y + k / 2) * W + 2 * x + k % 2) * C + ca + c]);
-            db[j] = f2bf(s);
+            float sacc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int k = 0; k < 4; k++) {
+                const uint4 q = *reinterpret_cast<const uint4 *>(
+                    dx + (((int64_t)n * H + 2 * y + k / 2) * W + 2 * x + k % 2) * C + ca +
+                    8 * c8);
+                const uint16_t *qv = reinterpret_cast<const uint16_t *>(&q);
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    sacc[e] += bf2f(qv[e]);
+            }
+            uint4 o;
+            uint16_t *ov = reinterpret_cast<uint16_t *>(&o);
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                ov[e] = f2bf(sacc[e]);
+            *reinterpret_cast<uint4 *>(db + ((((int64_t)n * H2 + y) * W2 + x) * cb + 8 * c8)) = o;
         }
     }
 }
@@ -662,29 +699,49 @@ __global__ __launch_bounds__(256) void final_bwd_kernel(const uint16_t *f, const
                                                         int C, const float *g, int64_t P,
                                                         uint16_t *df, float *partials)
 {
-    extern __shared__ float shf[];   // [256][C + 1]
-    for (int c = 0; c <= C; c++)
-        shf[threadIdx.x * (C + 1) + c] = 0.0f;
-    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < P;
-         p += (int64_t)gridDim.x * 256) {
+    // thread t takes the 8 channels 8 cg .. 8 cg + 7 (cg = t % g8, g8 = C / 8;
+    // the stride keeps cg fixed) of pixels with 16-byte loads and stores
+    __shared__ float shf[256][9];
+    const int t = threadIdx.x, g8 = C / 8, cg = t % g8;
+    float s[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float wc[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        wc[k] = w[8 * cg + k];
+    const int64_t n = P * g8;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + t; i < n; i += (int64_t)gridDim.x * 256) {
+        const int64_t p = i / g8;
         const float gp = g[p];
-        for (int c = 0; c < C; c++) {
-            const float fv = bf2f(f[p * C + c]);
-            df[p * C + c] = fv > 0.0f ? f2bf(gp * w[c]) : (uint16_t)0;
-            shf[threadIdx.x * (C + 1) + c] += gp * fv;
+        const uint4 v = *reinterpret_cast<const uint4 *>(f + p * C + 8 * cg);
+        const uint32_t in[4] = {v.x, v.y, v.z, v.w};
+        uint32_t out[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float f0 = __uint_as_float(in[k] << 16);
+            const float f1 = __uint_as_float(in[k] & 0xffff0000u);
+            const uint32_t d0 = f0 > 0.0f ? (uint32_t)f2bf(gp * wc[2 * k]) : 0u;
+            const uint32_t d1 = f1 > 0.0f ? (uint32_t)f2bf(gp * wc[2 * k + 1]) : 0u;
+            out[k] = d0 | (d1 << 16);
+            s[2 * k] += gp * f0;
+            s[2 * k + 1] += gp * f1;
         }
-        shf[threadIdx.x * (C + 1) + C] += gp;
+        *reinterpret_cast<uint4 *>(df + p * C + 8 * cg) = make_uint4(out[0], out[1], out[2],
+                                                                     out[3]);
+        if (cg == 0)
+            s[8] += gp;
     }
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+        shf[t][k] = s[k];
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int c = 0; c <= C; c++)
-                shf[threadIdx.x * (C + 1) + c] += shf[(threadIdx.x + o) * (C + 1) + c];
-        __syncthreads();
+    // channel c (and the bias term, c = C) summed over its threads in order
+    for (int c = t; c <= C; c += 256) {
+        const int q = c < C ? c / 8 : 0, k = c < C ? c % 8 : 8;
+        float tot = 0.0f;
+        for (int i = q; i < 256; i += g8)
+            tot += shf[i][k];
+        partials[(int64_t)blockIdx.x * (C + 1) + c] = tot;
     }
-    if (threadIdx.x == 0)
-        for (int c = 0; c <= C; c++)
-            partials[(int64_t)blockIdx.x * (C + 1) + c] = shf[c];
 }
 
 __global__ void adam_kernel(float *w, float *m, float *v, const float *g, int64_t n,
@@ -922,8 +979,7 @@ int backward(ctbb_unet *u, int batch, double *flops)
     // final layer
     const int nblk = (int)std::min<int64_t>((P0 + 255) / 256, 1024);
     Conv &last = u->convs.back();
-    hipLaunchKernelGGL(final_bwd_kernel, dim3(nblk), dim3(256),
-                       256 * (u->base + 1) * sizeof(float), u->stream, last.out,
+    hipLaunchKernelGGL(final_bwd_kernel, dim3(nblk), dim3(256), 0, u->stream, last.out,
                        u->params + u->final_w, u->base, u->gout, P0, last.dout, u->fpart);
     hipLaunchKernelGGL(sum_chunks_kernel, dim3(1), dim3(256), 0, u->stream, u->fpart,
                        (int64_t)nblk, (int64_t)(u->base + 1), u->grad + u->final_w);
