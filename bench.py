@@ -59,7 +59,7 @@ def parse():
     ap.add_argument("--config", default="config4")
     ap.add_argument("--tile", type=int, default=0, help="batch tile (0: library default)")
     ap.add_argument("--fp16", action="store_true", help="FP16 matrix for the timed sweeps")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of each cpu_baseline sample (0 = skip)")
     ap.add_argument("--ttt", type=int, default=1,
                     help="1: measure time to tolerance (needs the committed reference)")
@@ -88,6 +88,20 @@ def measured_traffic(workload_key: str, kernel: str):
         if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
             return float(parts[2])
     return None
+
+
+def ssm_build_flop(workload_key: str):
+    """FP64 FLOP of one matrix build (both passes) from the committed
+    rocprofv3 counter summary of this workload, or None."""
+    f = ROOT / "profiles" / f"ssm_build_{workload_key}_counters.txt"
+    if not f.exists():
+        return None
+    total = 0.0
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if parts and parts[0].startswith("ssm_build_kernel<") and len(parts) >= 6:
+            total += float(parts[5])
+    return total or None
 
 
 def cpu_topology() -> str:
@@ -365,6 +379,7 @@ def main():
     else:
         cost = (ttt or {}).get("plain_fp32", {}).get("cost")
     build_bytes = nnz * 8 + g.nx * g.ny * g.n_views_sm * 4
+    build_flop = ssm_build_flop(args.config)
     out = {
         "metric": METRIC,
         "value": value,
@@ -414,6 +429,12 @@ def main():
                 "ms": build_ms, "rays": rays, "rays_per_s": rays / (build_ms * 1e-3),
                 "bytes_written": build_bytes,
                 "hbm_write_frac": build_bytes / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "fp64_flop": build_flop,
+                "fp64_tflops": build_flop / (build_ms * 1e-3) / 1e12 if build_flop else None,
+                "fp64_valu_frac": (build_flop / (build_ms * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS
+                                   if build_flop else None),
+                "fp64_flop_source": "profiles/ssm_build_%s_counters.txt" % args.config
+                                    if build_flop else None,
             },
             "cpu_baseline_all_cores": cpu_all,
             "cost_after": cost,
