@@ -1478,13 +1478,39 @@ __global__ void rm_fill_kernel(const int64_t *col_ptr, const int32_t *idx,
 
 // Sort every ray's entries by column (the fill order is arbitrary): one
 // wavefront per ray.
-__global__ __launch_bounds__(256) void rm_sort_kernel(const int64_t *rptr, int64_t nrays,
+__global__ __launch_bounds__(256) void rm_sort_kernel(const int64_t *rptr,
+                                                      const int64_t *prptr, int64_t nrays,
                                                       uint32_t *rent, float *rval)
 {
     const int lane = threadIdx.x & 63;
     for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; q < nrays;
          q += (int64_t)gridDim.x * blockDim.x / 64)
-        wave_rank_sort(rptr[q], rptr[q + 1], rent, rval, lane);
+        wave_rank_sort(prptr[q], prptr[q] + (rptr[q + 1] - rptr[q]), rent, rval, lane);
+}
+
+// Ray counts rounded up to whole quads of entries.
+__global__ void rm_quad_count_kernel(uint32_t *cnt, int64_t nrays)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nrays;
+         q += (int64_t)gridDim.x * blockDim.x)
+        cnt[q] = (cnt[q] + kQuad - 1) / kQuad * kQuad;
+}
+
+// Padding of every ray to its quads: the ray's first column (any column of
+// the ray works) with value 0.
+__global__ void rm_pad_kernel(const int64_t *rptr, const int64_t *prptr, int64_t nrays,
+                              int64_t nr, uint32_t *rent, float *rval)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nrays;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t lo = prptr[q], n = rptr[q + 1] - rptr[q], hi = prptr[q + 1];
+        const uint32_t col = n > 0 ? (rent[lo] & kRmColMask) : 0u;
+        const uint32_t e = col | ((uint32_t)(q % nr) << kRmColBits);
+        for (int64_t p = lo + n; p < hi; p++) {
+            rent[p] = e;
+            rval[p] = 0.0f;
+        }
+    }
 }
 
 __global__ void rm_group_kernel(const int64_t *rptr, int64_t ngroups, int64_t nr,
@@ -1498,9 +1524,10 @@ __global__ void rm_group_kernel(const int64_t *rptr, int64_t ngroups, int64_t nr
 // Sample-driven FP64 projection.  One workgroup per (data view, kScatterCh
 // channels), all rows: for each slice k whose view window holds the view,
 // SSM view i = view - (c_k - n_sm/2); wavefront w takes channels c0+w,
-// c0+w+4, ..., reads a channel's entries 64 at a time, forms
-// x[col + nxy k] * a in FP64 per lane, sums runs of equal rows with
-// shuffles and adds each run into an FP64 LDS tile.  Fixed order, no
+// c0+w+4, ..., reads a channel's entries as quads (every ray padded to a
+// multiple of four entries, one ray's quad per lane), forms
+// sum x[col + nxy k] * a over the quad in FP64 per lane, sums runs of equal
+// rows across lanes with DPP moves and adds each run into an FP64 LDS tile.  Fixed order, no
 // atomics.  MODE 0: yhat = A x.  MODE 1: e = (float)(y - A x) (if e) and
 // one partial of w (y - A x)^2 per workgroup.  LDS: kScatterCh * nr doubles.
 constexpr int kProjUnroll = 4;
@@ -1544,27 +1571,36 @@ __global__ __launch_bounds__(kSweepBlock) void project_gather_kernel(
         for (int64_t c = c0 + wave; c < c1; c += kWaves) {
             const int64_t lo = vc_ptr[i * nch + c], hi = vc_ptr[i * nch + c + 1];
             double *accc = pacc + (c - c0) * nr;
-            // kProjUnroll chunks of 64 entries in flight, applied in order
-            for (int64_t p0 = lo; p0 < hi; p0 += 64 * kProjUnroll) {
-                uint32_t e0[kProjUnroll];
-                float av[kProjUnroll];
+            // rays are padded to quads of entries (one ray's four entries per
+            // lane); kProjUnroll chunks of 64 quads in flight, applied in order
+            const int64_t qlo = lo / kQuad, qhi = hi / kQuad;
+            const uint4 *rent4 = reinterpret_cast<const uint4 *>(rent);
+            const float4 *rval4 = reinterpret_cast<const float4 *>(rval);
+            for (int64_t p0 = qlo; p0 < qhi; p0 += 64 * kProjUnroll) {
+                uint4 e0[kProjUnroll];
+                float4 av[kProjUnroll];
                 bool ok[kProjUnroll];
 #pragma unroll
                 for (int u = 0; u < kProjUnroll; u++) {
                     const int64_t p = p0 + u * 64 + lane;
-                    ok[u] = p < hi;
-                    e0[u] = ok[u] ? rent[p] : 0u;
-                    av[u] = ok[u] ? rval[p] : 0.0f;
+                    ok[u] = p < qhi;
+                    e0[u] = ok[u] ? rent4[p] : make_uint4(0, 0, 0, 0);
+                    av[u] = ok[u] ? rval4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
                 double d[kProjUnroll];
 #pragma unroll
-                for (int u = 0; u < kProjUnroll; u++)
-                    d[u] = ok[u] ? (double)xk[e0[u] & kRmColMask] * (double)av[u] : 0.0;
+                for (int u = 0; u < kProjUnroll; u++) {
+                    double t = (double)xk[e0[u].x & kRmColMask] * (double)av[u].x;
+                    t += (double)xk[e0[u].y & kRmColMask] * (double)av[u].y;
+                    t += (double)xk[e0[u].z & kRmColMask] * (double)av[u].z;
+                    t += (double)xk[e0[u].w & kRmColMask] * (double)av[u].w;
+                    d[u] = ok[u] ? t : 0.0;
+                }
 #pragma unroll
                 for (int u = 0; u < kProjUnroll; u++) {
-                    if (p0 + u * 64 >= hi)
+                    if (p0 + u * 64 >= qhi)
                         break;
-                    const int row = ok[u] ? (int)(e0[u] >> kRmColBits) : -1;
+                    const int row = ok[u] ? (int)(e0[u].x >> kRmColBits) : -1;
                     const uint64_t head = run_heads(row, lane);
                     const double t = run_sum(d[u], head, lane);
                     if (ok[u] && run_end(head, lane))
@@ -1688,7 +1724,7 @@ int ensure_ray_major(const ctbb_ssm *mc)
     const int64_t nrays = g->n_views_sm * g->n_channels * g->n_rows;
     const int64_t ngroups = g->n_views_sm * g->n_channels;
     uint32_t *cnt = nullptr;
-    int64_t *sums = nullptr, *rptr = nullptr;
+    int64_t *sums = nullptr, *rptr = nullptr, *prptr = nullptr;
     unsigned long long *cursor = nullptr;
     const int64_t nchunk = (nrays + kScanChunk - 1) / kScanChunk;
     int rc = CTBB_OK;
@@ -1696,6 +1732,7 @@ int ensure_ray_major(const ctbb_ssm *mc)
         if (cnt) (void)hipFree(cnt);
         if (sums) (void)hipFree(sums);
         if (rptr) (void)hipFree(rptr);
+        if (prptr) (void)hipFree(prptr);
         if (cursor) (void)hipFree(cursor);
     };
     auto drop = [&]() {
@@ -1709,9 +1746,8 @@ int ensure_ray_major(const ctbb_ssm *mc)
     if ((rc = dev_alloc_n(&cnt, nrays, "ray counts")) ||
         (rc = dev_alloc_n(&sums, nchunk, "chunk sums")) ||
         (rc = dev_alloc_n(&rptr, nrays + 1, "ray pointers")) ||
-        (rc = dev_alloc_n(&m->rm_vc, ngroups + 1, "ray-major group starts")) ||
-        (rc = dev_alloc_n(&m->rm_ent, std::max<int64_t>(1, m->nnz), "ray-major entries")) ||
-        (rc = dev_alloc_n(&m->rm_val, std::max<int64_t>(1, m->nnz), "ray-major values"))) {
+        (rc = dev_alloc_n(&prptr, nrays + 1, "padded ray pointers")) ||
+        (rc = dev_alloc_n(&m->rm_vc, ngroups + 1, "ray-major group starts"))) {
         cleanup();
         drop();
         return rc;
@@ -1745,6 +1781,37 @@ int ensure_ray_major(const ctbb_ssm *mc)
         e = hipMemcpy(rptr + nrays, &run, 8, hipMemcpyHostToDevice);
     if (e == hipSuccess && run != m->nnz)
         e = hipErrorUnknown;
+    // every ray padded to whole quads: the same scan over the rounded counts
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(rm_quad_count_kernel, dim3(4096), dim3(256), 0, 0, cnt, nrays);
+        hipLaunchKernelGGL(chunk_sum_kernel, dim3((unsigned)nchunk), dim3(256), 0, 0,
+                           cnt, nrays, sums);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(hs.data(), sums, nchunk * 8, hipMemcpyDeviceToHost);
+    int64_t prun = 0;
+    for (int64_t i = 0; i < nchunk; i++) {
+        const int64_t v = hs[i];
+        hs[i] = prun;
+        prun += v;
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(sums, hs.data(), nchunk * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(chunk_scan_kernel, dim3((unsigned)nchunk), dim3(256), 0, 0,
+                           cnt, nrays, sums, prptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(prptr + nrays, &prun, 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess &&
+        ((rc = dev_alloc_n(&m->rm_ent, std::max<int64_t>(1, prun), "ray-major entries")) ||
+         (rc = dev_alloc_n(&m->rm_val, std::max<int64_t>(1, prun), "ray-major values")))) {
+        cleanup();
+        drop();
+        return rc;
+    }
     if (e == hipSuccess) {
         (void)hipFree(cnt);
         cnt = nullptr;
@@ -1754,15 +1821,17 @@ int ensure_ray_major(const ctbb_ssm *mc)
             drop();
             return rc;
         }
-        e = hipMemcpy(cursor, rptr, nrays * 8, hipMemcpyDeviceToDevice);
+        e = hipMemcpy(cursor, prptr, nrays * 8, hipMemcpyDeviceToDevice);
     }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(rm_fill_kernel, dim3(gcol), dim3(256), 0, 0, m->col_ptr,
                            m->idx, m->val, m->val16, m->col_scale, m->ncol, g->n_rows,
                            cursor, m->rm_ent, m->rm_val);
-        hipLaunchKernelGGL(rm_sort_kernel, dim3(16384), dim3(256), 0, 0, rptr, nrays,
+        hipLaunchKernelGGL(rm_sort_kernel, dim3(16384), dim3(256), 0, 0, rptr, prptr, nrays,
                            m->rm_ent, m->rm_val);
-        hipLaunchKernelGGL(rm_group_kernel, dim3(4096), dim3(256), 0, 0, rptr, ngroups,
+        hipLaunchKernelGGL(rm_pad_kernel, dim3(8192), dim3(256), 0, 0, rptr, prptr, nrays,
+                           g->n_rows, m->rm_ent, m->rm_val);
+        hipLaunchKernelGGL(rm_group_kernel, dim3(4096), dim3(256), 0, 0, prptr, ngroups,
                            g->n_rows, m->rm_vc);
         e = hipGetLastError();
     }

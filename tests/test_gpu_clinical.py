@@ -130,27 +130,50 @@ def test_fullsize_sweep_lowers_cost_and_resync_is_exact(gpu, make_scan, name):
     m.free()
 
 
-def test_config2_fp16_resync_against_fp32(gpu, make_scan):
+def test_config2_fp16_resync_against_fp32_and_fp64(gpu, make_scan):
     """Config 2's precision split (FP16 matrix with per-column scales, FP64
-    resync every 2 sweeps) against the FP32 plain run of the same data, over
-    4 sweeps."""
-    s, m, r, raw, w, c0, st = _fullsize(gpu, make_scan, "config1", sweeps=4)
-    x32 = r.volume().astype(np.float64)
-    c32 = r.cost()
-    r.free()
-    m.free()
-    s2, m2, r2, raw2, w2, c0b, st2 = _fullsize(gpu, make_scan, "config2", fp16=True,
-                                                sweeps=4, resync=2)
-    assert np.array_equal(raw, raw2)
-    x16 = r2.volume().astype(np.float64)
-    c16 = r2.cost()
-    rms = np.sqrt(np.mean((x16 - x32) ** 2)) / np.sqrt(np.mean(x32 ** 2))
-    mx = np.abs(x16 - x32).max()
+    resync every 2 sweeps) against the FP32 plain run of the same data and
+    against the FP64 state (x, e and the sums in FP64: the GPU mode that
+    matches the CPU FP64 oracle to ~1e-15 in tests/test_gpu_fp64.py, standing
+    in for it at this size), all in the same order over 4 sweeps: volume
+    max / RMS error and final cost."""
+    def run(name, fp16=False, resync=0, fp64=0):
+        s = make_scan(name)
+        raw, w = s.synthesize(use_gpu=True)
+        m = gpu.SSM.build(s.geom)
+        if fp16:
+            m.to_fp16()
+        r = gpu.Recon(m, raw, w, s.slice_index,
+                      s.opts(resync_every=resync, state_fp64=fp64))
+        c0 = r.cost()
+        st = r.iterate(4)
+        x = r.volume64() if fp64 else r.volume().astype(np.float64)
+        c = r.cost()
+        r.free()
+        m.free()
+        return raw, x, c0, c, st
+
+    raw64, x64, _, c64, _ = run("config1", fp64=1)
+    raw32, x32, _, c32, _ = run("config1")
+    raw16, x16, c0, c16, st16 = run("config2", fp16=True, resync=2)
+    assert np.array_equal(raw32, raw16) and np.array_equal(raw32, raw64)
+    hu = 1.93e-4
+    res = {}
+    for tag, x, c in (("FP32 plain", x32, c32), ("FP16 + resync", x16, c16)):
+        d = x - x64
+        res[tag] = (np.sqrt(np.mean(d ** 2)) / np.sqrt(np.mean(x64 ** 2)), np.abs(d).max(),
+                    (c - c64) / c64)
+        print(f"[config2] {tag} vs FP64 state after 4 sweeps: rel RMS {res[tag][0]:.3e}, "
+              f"max |dx| {res[tag][1]:.3e} /cm ({res[tag][1] / hu:.2f} HU), cost rel "
+              f"{res[tag][2]:+.2e}", flush=True)
+    d = x16 - x32
+    rms = np.sqrt(np.mean(d ** 2)) / np.sqrt(np.mean(x32 ** 2))
+    mx = np.abs(d).max()
     print(f"[config2] FP16+resync vs FP32 after 4 sweeps: rel RMS {rms:.3e}, max "
-          f"|dx| {mx:.3e} /cm ({mx / 1.93e-4:.2f} HU), cost {c16:.8e} vs {c32:.8e} "
-          f"(rel {(c16 - c32) / c32:+.2e}); resync {st2.resync_ms:.0f} ms", flush=True)
-    assert c16 < c0b
-    assert rms <= 1e-2
+          f"|dx| {mx:.3e} /cm ({mx / hu:.2f} HU), cost {c16:.8e} vs {c32:.8e} "
+          f"(rel {(c16 - c32) / c32:+.2e}); resync {st16.resync_ms:.0f} ms", flush=True)
+    assert c16 < c0
+    assert res["FP32 plain"][0] <= 1e-4          # the FP32 state's bound
+    assert abs(res["FP32 plain"][2]) <= 1e-5
+    assert rms <= 1e-2 and res["FP16 + resync"][0] <= 1e-2
     assert abs(c16 - c32) <= 1e-3 * c32
-    r2.free()
-    m2.free()
