@@ -25,9 +25,10 @@
 // step is Jacobi, along a column Gauss-Seidel in group order; with tile >=
 // max(nx, ny) batches are single columns and the order is the reference's.
 // By default the curvature of a voxel's update is the batch surrogate
-// sum_i w_i a_i s_i (s_i: the batch's summed matrix values on ray i), which
-// majorises the batch's data term, so every step lowers the cost; for single
-// columns it is the reference's sum_i w_i a_i^2.  The oracle replays exactly
+// sum_r w_r a_r S_r (S_r: the summed matrix values, on data sample r, of
+// every voxel of the step -- all the batch's columns, every slice of the
+// group), which majorises the step's data term, so every step lowers the
+// cost; single-column batches keep the reference's sum_i w_i a_i^2.  The oracle replays exactly
 // this order and curvature.
 //
 // FP64 is used for the per-voxel sums of partials and the penalty solve, the
@@ -471,8 +472,9 @@ struct StepArgs {
 // (each entry is loaded once per pass and gathered at each member's sample);
 // neighbouring lanes read neighbouring samples of the error sinogram, so a
 // 64-lane gather touches few cache lines.  Lanes form a * (w e) at sample
-// idx + slice offset (and, on the first sweep, w a^2); runs of equal entry
-// keys (view in block, rank) are summed with shuffles and the run's last
+// idx + slice offset (and, on the first sweep, w a S, the surrogate
+// curvature); runs of equal entry keys (view in block, rank) are summed
+// with DPP lane moves and the run's last
 // lane adds into the wavefront's LDS accumulator of that key (one wavefront
 // per chunk, in entry order: deterministic).  Per member, each lane adds its
 // rank's kViewBlock accumulators in view order and writes the partial of
