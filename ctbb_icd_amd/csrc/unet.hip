@@ -877,7 +877,7 @@ int run_conv_dgrad(ctbb_unet *u, const Conv &c, int batch, uint16_t *dst, double
     return CTBB_OK;
 }
 
-constexpr int64_t kWgradChunk = 4096;
+constexpr int64_t kWgradChunk = 2048;
 
 // The tiled weight-gradient kernel applies (image rows of a multiple of 32
 // pixels, 32-channel blocks) and its chunk height: whole rows, about
