@@ -348,6 +348,111 @@ __global__ __launch_bounds__(256) void conv3x3_wgrad_kernel(ConvIn in, const uin
     }
 }
 
+// Weight gradient of a convolution with 8 input channels (the first
+// layer): part[chunk][tap][co][ci] like conv3x3_wgrad_kernel, but the MFMA's
+// N dimension is (tap, input channel) -- 72 columns in three 32-column
+// tiles -- so one pass over a chunk's pixels serves all 9 taps and each dY
+// operand feeds three MFMAs.  Four wavefronts take quarters of the chunk;
+// their tiles are added in LDS in wavefront order.  Grid: chunks x nco.
+__global__ __launch_bounds__(256) void conv3x3_wgrad_c8_kernel(ConvIn in, const uint16_t *dy,
+                                                               int N, int H, int W, int cout,
+                                                               int64_t chunk, int nco,
+                                                               float *part)
+{
+    __shared__ float red[4][16][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int co_t = (int)(blockIdx.x % nco);
+    const int64_t ck = blockIdx.x / nco;
+    const int64_t P = (int64_t)N * H * W;
+    const int64_t q4 = chunk / 4;
+    const int64_t p0 = ck * chunk + wave * q4;
+    const int64_t p1 = min(p0 + q4, P);
+    const int r = lane & 31, kg = lane >> 5;
+    const int co = co_t * 32 + r;
+    const bool cov = co < cout;
+    bool colv[3];
+    int ci_c[3], dyo[3], dxo[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        const int c = t * 32 + r;
+        colv[t] = c < 72;
+        const int tap = colv[t] ? c / 8 : 0;
+        ci_c[t] = c % 8;
+        dyo[t] = tap / 3 - 1;
+        dxo[t] = tap % 3 - 1;
+    }
+    v16f acc[3] = {{0}, {0}, {0}};
+    // (n, y, x) of this lane's first pixel p0 + 4 kg, advanced incrementally
+    int64_t pq = p0 + 4 * kg;
+    int n0 = 0, y0 = 0, x0 = 0;
+    if (pq < P) {
+        n0 = (int)(pq / ((int64_t)H * W));
+        const int rem = (int)(pq - (int64_t)n0 * H * W);
+        y0 = rem / W;
+        x0 = rem - y0 * W;
+    }
+    for (int64_t p = p0; p < p1; p += 8) {
+        v4s av, bv[3];
+        int n = n0, y = y0, x = x0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t pp = p + 4 * kg + j;
+            uint16_t a = 0, bb[3] = {0, 0, 0};
+            if (pp < p1) {
+                if (cov)
+                    a = dy[pp * cout + co];
+#pragma unroll
+                for (int t = 0; t < 3; t++)
+                    if (colv[t])
+                        bb[t] = load_in1(in, n, y + dyo[t], x + dxo[t], H, W, ci_c[t]);
+            }
+            av[j] = (short)a;
+#pragma unroll
+            for (int t = 0; t < 3; t++)
+                bv[t][j] = (short)bb[t];
+            if (++x == W) {
+                x = 0;
+                if (++y == H) {
+                    y = 0;
+                    n++;
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 3; t++)
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv[t], acc[t], 0, 0, 0);
+        // next group: 8 pixels on from this lane's first pixel
+        x0 += 8;
+        while (x0 >= W) {
+            x0 -= W;
+            if (++y0 == H) {
+                y0 = 0;
+                n0++;
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+            red[wave][q][lane] = acc[t][q];
+        __syncthreads();
+        if (wave == 0) {
+            const int c = t * 32 + r;
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int oc = co_t * 32 + 8 * (q >> 2) + 4 * kg + (q & 3);
+                if (oc < cout && c < 72) {
+                    const float v = ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) +
+                                    red[3][q][lane];
+                    part[((ck * 9 + c / 8) * cout + oc) * 8 + c % 8] = v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Tiled weight gradient: part[chunk][tap][co][ci] = sum over the chunk's
 // pixels of dY[p][co] in[p + o(tap)][ci], for a 32 x 32 (co, ci) block and
 // all 9 taps per workgroup.  A chunk is `crows` consecutive image rows of
@@ -913,6 +1018,10 @@ int run_conv_wgrad(ctbb_unet *u, const Conv &c, int batch, double *flops)
         hipLaunchKernelGGL(conv3x3_wgrad_tiled, dim3((unsigned)(nchunk * nco * nci)), dim3(256),
                            0, u->stream, c.in, c.dout, batch, Hl, Wl, c.cin, c.cout, crows,
                            nco, nci, u->wpart);
+    else if (c.cin == 8 && !c.in.b)
+        hipLaunchKernelGGL(conv3x3_wgrad_c8_kernel, dim3((unsigned)(nchunk * nco)), dim3(256),
+                           0, u->stream, c.in, c.dout, batch, Hl, Wl, c.cout, kWgradChunk,
+                           nco, u->wpart);
     else
         hipLaunchKernelGGL(conv3x3_wgrad_kernel, dim3((unsigned)(nchunk * 9 * nco * nci)),
                            dim3(256), 0, u->stream, c.in, c.dout, batch, Hl, Wl, c.cin, c.cout,
