@@ -304,6 +304,8 @@ def _main():
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--drop-ends", type=int, default=2,
+                    help="slices at either end of a training volume left out of the loss")
     ap.add_argument("--fbp", action="store_true",
                     help="also time ICD from the GPU FBP start (same matrix as the warm run)")
     ap.add_argument("--fp16-warm", action="store_true",
@@ -384,7 +386,7 @@ def _main():
         ssm.free()
         net = UNet(4, 32, a.batch, g.ny, g.nx, seed=11)
         t = time.perf_counter()
-        hist = train(net, pairs, a.epochs, a.lr, log=log)
+        hist = train(net, pairs, a.epochs, a.lr, log=log, drop_ends=a.drop_ends)
         t_train = time.perf_counter() - t
         ms, flops = net.last_timing()
         log(f"[warmstart] training {t_train:.1f} s; last step {ms:.2f} ms, "
@@ -399,6 +401,7 @@ def _main():
                                           seeds=[a.train_seed0, a.train_seed0 + a.train],
                                           early=a.early, target_its=a.target_its,
                                           target_start=a.target_start, epochs=a.epochs,
+                                          drop_ends=a.drop_ends,
                                           lr=a.lr, final_loss=hist[-1],
                                           train_seconds=t_train))
         del pairs
