@@ -70,6 +70,82 @@ inline int64_t slice_offset(const ctbb_geom *g, int64_t central)
     return (central - g->n_views_sm / 2) * g->n_channels * g->n_rows;
 }
 
+// ---------------------------------------------------------------------------
+// Wavefront run sums (segmented inclusive scans over lanes whose keys are
+// sorted), shared by the sweep, projection and backprojection kernels.
+
+// DPP lane moves: row_shr:n inside rows of 16 lanes (0 from outside the
+// row), row_bcast:15 / row_bcast:31 (lane 15 of each row to the next row /
+// lane 31 to rows 2 and 3), wave_shr:1 (lane l - 1; lane 0 keeps `old`).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_mov(float v)
+{
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xf,
+                                               false);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Inclusive sums over runs of equal keys (segmented scan): lane l adds the
+// values of lanes [start, l], start = the last head at or before l.  Four
+// in-row shifts, then the carries across rows (row 0 -> 1 and 2 -> 3, then
+// rows 0-1 -> 2 and 3); steps no lane needs (every run shorter) are
+// skipped, wave-uniformly.
+template <typename T>
+__device__ __forceinline__ T run_sum(T v, uint64_t head, int lane)
+{
+    const uint64_t upto = head & (~0ull >> (63 - lane));
+    const int start = 63 - __clzll((unsigned long long)upto);
+    const int dist = lane - start;
+    if (!__ballot(dist >= 1))
+        return v;
+    T t;
+    t = dpp_mov<0x111, 0xf>(v);
+    if (dist >= 1) v += t;
+    if (__ballot(dist >= 2)) {
+        t = dpp_mov<0x112, 0xf>(v);
+        if (dist >= 2) v += t;
+        if (__ballot(dist >= 4)) {
+            t = dpp_mov<0x114, 0xf>(v);
+            if (dist >= 4) v += t;
+            if (__ballot(dist >= 8)) {
+                t = dpp_mov<0x118, 0xf>(v);
+                if (dist >= 8) v += t;
+            }
+        }
+    }
+    // carries across rows: lanes whose run began in an earlier row
+    const int row0 = lane & ~15;
+    if (__ballot(start < row0)) {
+        t = dpp_mov<0x142, 0xa>(v);
+        if ((lane & 16) && start < row0) v += t;
+        t = dpp_mov<0x143, 0xc>(v);
+        if ((lane & 32) && start < 32) v += t;
+    }
+    return v;
+}
+
+// Head mask of a wavefront whose lanes hold sorted keys (invalid lanes, at
+// the tail, carry key -1) and run-end test for lane `lane`.
+__device__ __forceinline__ uint64_t run_heads(int key, int lane)
+{
+    const int prev = __builtin_amdgcn_update_dpp(-2, key, 0x138, 0xf, 0xf, false);
+    return __ballot(lane == 0 || prev != key);
+}
+
+__device__ __forceinline__ bool run_end(uint64_t head, int lane)
+{
+    return lane == 63 || ((head >> (lane + 1)) & 1ull);
+}
+
 }  // namespace ctbb
 
 // Stored system matrix, device resident, compressed sparse column.
