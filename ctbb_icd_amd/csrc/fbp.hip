@@ -60,102 +60,159 @@ __global__ __launch_bounds__(kFbpBlock) void fbp_filter_kernel(
     }
 }
 
-constexpr int kFbpUnroll = 8;
-
-// Squared in-plane distance of a column at (px, py) from the nominal source
-// of SSM view v.
-__device__ __forceinline__ double fbp_l2(int64_t v, int64_t nsm, double dtheta, double R,
-                                         double px, double py)
+// floor(a / d) for a, d >= 1 below 2^31 with a quotient below 2^20: the
+// FP32 estimate a * (1/d) is within 2^20 * 4 * 2^-24 < 1 of a / d, so one
+// correction step makes it exact.
+__device__ __forceinline__ uint32_t fbp_div(uint32_t a, uint32_t d, float inv)
 {
-    const double th = ((double)v - 0.5 * (double)nsm) * dtheta;
-    const double dx = R * cos(th) - px, dy = R * sin(th) - py;
-    return dx * dx + dy * dy;
+    uint32_t q = (uint32_t)((float)a * inv);
+    if (q * d > a)
+        q--;
+    else if ((q + 1) * d <= a)
+        q++;
+    return q;
 }
 
-// One wavefront per FOV column; lane l takes slices l, l + 64, ...  The
-// column's entries are read once per 64 slices (every lane reads the same
-// entry); each lane gathers Q at its slice's sample.  Sums in FP64.
-// nvr = n_channels * n_rows (samples per view), nturn = views per turn.
+__device__ __forceinline__ double fbp_readlane(double v, int l)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// One wavefront per FOV column.  Lanes take 64 consecutive entries of the
+// column (entries run view by view, so a view's two or three entries sit in
+// neighbouring lanes and their samples in one or two cache lines); for each
+// of kFbpSlices slices at a time every lane gathers Q at its entry's sample,
+// runs of one view are summed with DPP lane moves (a view cut by the chunk
+// boundary is carried into the next chunk), and the lane that ends a view
+// adds the view's weighted, footprint-normalised value to its slice sums;
+// the slice sums are added over the lanes at the end.  The column's entries
+// are read once per kFbpSlices slices.  Per view (once per chunk, not per
+// slice): 1 / (L^2 * footprint sum) from the source table src (R cos th_v,
+// R sin th_v per SSM view) and the redundancy weight over the column's whole
+// view range, which every slice whose range is not clipped by the sinogram's
+// ends uses.  Sums in FP64.  nvr = n_channels * n_rows (samples per view),
+// nturn = views per turn; inv_* = their FP32 reciprocals.
+constexpr int kFbpSlices = 8;
+
 template <bool FP16>
 __global__ __launch_bounds__(kFbpBlock) void fbp_backproject_kernel(
     const int64_t *col_ptr, const int32_t *idx, const float *val, const __half *val16,
-    const float *col_scale, const int32_t *view_off, const int32_t *cols, int64_t ncols,
-    int64_t nx, int64_t nxy, int64_t nz, int64_t nsm, double cx, double cy, double vx,
-    double vy, double R, double dtheta, const int64_t *slice_off,
-    int64_t ds, int64_t nvr, int64_t nturn, const float *q, float *x)
+    const float *col_scale, const int32_t *cols, int64_t ncols, int64_t nx, int64_t nxy,
+    int64_t nz, double cx, double cy, double vx, double vy, const double *src, int64_t nsm,
+    const int64_t *slice_off, int64_t ds, uint32_t nvr, float inv_nvr, uint32_t nturn,
+    float inv_nturn, const float *q, float *x)
 {
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = (int64_t)gridDim.x * (kFbpBlock / 64);
-    for (int64_t i = ((int64_t)blockIdx.x * kFbpBlock + threadIdx.x) / 64; i < ncols;
-         i += nwaves) {
+    const int ndv = (int)(ds / nvr);   // data views
+    for (int64_t i = __builtin_amdgcn_readfirstlane(
+             (int)(((int64_t)blockIdx.x * kFbpBlock + threadIdx.x) / 64));
+         i < ncols; i += nwaves) {
         const int64_t col = cols[i];
         const int64_t ix = col % nx, iy = col / nx;
         const double px = (ix - cx) * vx, py = (iy - cy) * vy;
         const int64_t b = col_ptr[col], len = col_ptr[col + 1] - b;
-        const int32_t *vo = view_off + col * nsm;
-        const float sc = FP16 ? col_scale[col] : 1.0f;
-        // views of the column's first and last entry (entries run view by view)
-        const int64_t fv = len > 0 ? (int64_t)idx[b] / nvr : 0;
-        const int64_t lv = len > 0 ? (int64_t)idx[b + len - 1] / nvr : -1;
-        for (int64_t k0 = 0; k0 < nz; k0 += 64) {
-            const int64_t k = k0 + lane;
-            const bool own = k < nz;
-            const int64_t off = own ? slice_off[k] : 0;
-            // the slice's views whose samples lie in the sinogram: off is a
-            // whole number of views, so the in-range views are a range
-            const int64_t offv = off / nvr;
-            const int64_t first = fv > -offv ? fv : -offv;
-            const int64_t last = lv < ds / nvr - 1 - offv ? lv : ds / nvr - 1 - offv;
-            // the column's entries run view by view; kFbpUnroll of them are
-            // loaded and gathered at once, then summed in entry order, a
-            // view's footprint sum closing where its entries end
-            double acc = 0.0, sa = 0.0, sq = 0.0, wsum = 0.0;
-            int64_t v = 0;
-            // closes view v: its weight 1 / n_v (sa > 0 puts v in [first, last])
-            auto close_view = [&]() {
-                const double wv =
-                    1.0 / (double)((last - v) / nturn + (v - first) / nturn + 1);
-                acc += wv * (sq / sa / fbp_l2(v, nsm, dtheta, R, px, py));
-                wsum += wv;
-            };
-            int64_t e1 = nsm > 1 ? (int64_t)vo[1] : len;
-            for (int64_t e0 = 0; e0 < len; e0 += kFbpUnroll) {
-                double av[kFbpUnroll], qv[kFbpUnroll];
-                bool ok[kFbpUnroll];
+        const double sc = FP16 ? (double)col_scale[col] : 1.0;
+        // views of the column's first and last entry
+        const int fv = len > 0 ? (int)fbp_div((uint32_t)idx[b], nvr, inv_nvr) : 0;
+        const int lv = len > 0 ? (int)fbp_div((uint32_t)idx[b + len - 1], nvr, inv_nvr) : -1;
+        for (int64_t k0 = 0; k0 < nz; k0 += kFbpSlices) {
+            int64_t off[kFbpSlices];
+            int first[kFbpSlices], last[kFbpSlices];
+            bool slice[kFbpSlices], clip[kFbpSlices];
 #pragma unroll
-                for (int u = 0; u < kFbpUnroll; u++) {
-                    const int64_t e = e0 + u;
-                    const bool in = e < len;
-                    const int64_t r = in ? (int64_t)idx[b + e] + off : -1;
-                    ok[u] = own && in && r >= 0 && r < ds;
-                    av[u] = in ? (FP16 ? (double)__half2float(val16[b + e]) * (double)sc
-                                       : (double)val[b + e])
-                               : 0.0;
-                    qv[u] = ok[u] ? (double)q[r] : 0.0;
+            for (int g = 0; g < kFbpSlices; g++) {
+                slice[g] = k0 + g < nz;
+                off[g] = slice[g] ? slice_off[k0 + g] : 0;
+                // the slice's views whose samples lie in the sinogram: off is
+                // a whole number of views, so they are a range
+                const int offv = (int)(off[g] / (int64_t)nvr);
+                first[g] = fv > -offv ? fv : -offv;
+                last[g] = lv < ndv - 1 - offv ? lv : ndv - 1 - offv;
+                if (!slice[g]) {   // no view is in an absent slice's range
+                    first[g] = 1;
+                    last[g] = 0;
                 }
-#pragma unroll
-                for (int u = 0; u < kFbpUnroll; u++) {
-                    const int64_t e = e0 + u;
-                    if (e >= len)
-                        break;
-                    while (e >= e1) {     // close view v (and any empty ones)
-                        if (sa > 0.0)
-                            close_view();
-                        sa = 0.0;
-                        sq = 0.0;
-                        v++;
-                        e1 = v + 1 < nsm ? (int64_t)vo[v + 1] : len;
-                    }
-                    if (ok[u]) {
-                        sa += av[u];
-                        sq += av[u] * qv[u];
-                    }
-                }
+                clip[g] = first[g] != fv || last[g] != lv;
             }
-            if (sa > 0.0)
-                close_view();
-            if (own)
-                x[col + nxy * k] = wsum > 0.0 ? (float)(2.0 * M_PI * acc / wsum) : 0.0f;
+            double acc[kFbpSlices], wsum[kFbpSlices];
+#pragma unroll
+            for (int g = 0; g < kFbpSlices; g++) {
+                acc[g] = 0.0;
+                wsum[g] = 0.0;
+            }
+            // the last view of the previous chunk when it continues here
+            double carry_a = 0.0, carry_q[kFbpSlices];
+#pragma unroll
+            for (int g = 0; g < kFbpSlices; g++)
+                carry_q[g] = 0.0;
+            for (int64_t c0 = 0; c0 < len; c0 += 64) {
+                const int64_t e = c0 + lane;
+                const bool in = e < len;
+                const int32_t r0 = in ? idx[b + e] : 0;
+                const double a = in ? (FP16 ? (double)__half2float(val16[b + e]) * sc
+                                            : (double)val[b + e])
+                                    : 0.0;
+                const int v = in ? (int)fbp_div((uint32_t)r0, nvr, inv_nvr) : -1;
+                // does this chunk's last view go on in the next chunk?
+                const bool cont =
+                    c0 + 64 < len && fbp_div((uint32_t)idx[b + c0 + 64], nvr, inv_nvr) ==
+                                         fbp_div((uint32_t)idx[b + c0 + 63], nvr, inv_nvr);
+                const uint64_t head = run_heads(v, lane);
+                const bool end = in && run_end(head, lane) && !(cont && lane == 63);
+                // lane 0 takes the carried part of its view
+                const double sa = run_sum(lane == 0 ? a + carry_a : a, head, lane);
+                // per view: f = 1 / (L^2 sa), and the redundancy weight over
+                // the unclipped range; both only where a view ends
+                const bool use = end && sa > 0.0 && v < nsm;
+                double f = 0.0, w0 = 0.0;
+                if (use) {
+                    const double dx = src[2 * v] - px, dy = src[2 * v + 1] - py;
+                    f = 1.0 / ((dx * dx + dy * dy) * sa);
+                    w0 = 1.0 / (double)(fbp_div((uint32_t)(lv - v), nturn, inv_nturn) +
+                                        fbp_div((uint32_t)(v - fv), nturn, inv_nturn) + 1);
+                }
+                double sq[kFbpSlices];
+#pragma unroll
+                for (int g = 0; g < kFbpSlices; g++) {
+                    // in the slice's view range <=> the sample is in the sinogram
+                    const int64_t rr = (int64_t)r0 + off[g];
+                    const bool ok = in && v >= first[g] && v <= last[g] && rr >= 0 && rr < ds;
+                    const double t = ok ? a * (double)q[rr] : 0.0;
+                    sq[g] = lane == 0 ? t + carry_q[g] : t;
+                }
+#pragma unroll
+                for (int g = 0; g < kFbpSlices; g++) {
+                    const double s = run_sum(sq[g], head, lane);
+                    if (use && v >= first[g] && v <= last[g]) {
+                        // weight 1 / n_v: every source angle of the range counts once
+                        const double wv =
+                            clip[g] ? 1.0 / (double)(fbp_div((uint32_t)(last[g] - v), nturn,
+                                                             inv_nturn) +
+                                                     fbp_div((uint32_t)(v - first[g]), nturn,
+                                                             inv_nturn) +
+                                                     1)
+                                    : w0;
+                        acc[g] += wv * (s * f);
+                        wsum[g] += wv;
+                    }
+                    carry_q[g] = cont ? fbp_readlane(s, 63) : 0.0;
+                }
+                carry_a = cont ? fbp_readlane(sa, 63) : 0.0;
+            }
+#pragma unroll
+            for (int g = 0; g < kFbpSlices; g++) {
+                double s1 = acc[g], s2 = wsum[g];
+                for (int o = 32; o > 0; o >>= 1) {
+                    s1 += __shfl_xor(s1, o, 64);
+                    s2 += __shfl_xor(s2, o, 64);
+                }
+                if (lane == 0 && slice[g])
+                    x[col + nxy * (k0 + g)] = s2 > 0.0 ? (float)(2.0 * M_PI * s1 / s2) : 0.0f;
+            }
         }
     }
 }
@@ -168,9 +225,11 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
     if (!m || !raw || !slice_index || !volume)
         return fail(CTBB_E_ARG, "null argument");
     const ctbb_geom *g = &m->g;
-    if (!m->view_off)
-        return fail(CTBB_E_STATE, "matrix without view offsets");
     const int64_t nch = g->n_channels, nr = g->n_rows, nsm = g->n_views_sm;
+    // the backprojection's FP32-estimated integer quotients (views) are exact
+    // below 2^20
+    if (nsm >= (1 << 20))
+        return fail(CTBB_E_ARG, "more than 2^20 SSM views");
     const int64_t nxy = g->nx * g->ny, nvox = nxy * g->nz, ds = data_size_of(g);
     if (nch * (int64_t)sizeof(float) > 65536)
         return fail(CTBB_E_ARG, "detector line of " + std::to_string(nch) +
@@ -195,6 +254,13 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
     const int64_t nturn = (int64_t)std::llround(2.0 * M_PI / g->tube_angle_increment);
     if (nturn < 1)
         return fail(CTBB_E_ARG, "tube angle increment above a turn");
+    // nominal source of each SSM view, (R cos th_v, R sin th_v)
+    std::vector<double> src(2 * nsm);
+    for (int64_t v = 0; v < nsm; v++) {
+        const double th = ((double)v - 0.5 * (double)nsm) * g->tube_angle_increment;
+        src[2 * v] = Rs * cos(th);
+        src[2 * v + 1] = Rs * sin(th);
+    }
     // FOV columns and slice offsets
     const double fov = (g->acquisition_fov / 2.0) * (g->acquisition_fov / 2.0);
     std::vector<int32_t> cols;
@@ -226,7 +292,7 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
     if ((rc = dev_alloc_n(&d_raw, ds, "raw data")) ||
         (rc = dev_alloc_n(&d_q, ds, "filtered sinogram")) ||
         (rc = dev_alloc_n(&d_x, nvox, "volume")) ||
-        (rc = dev_alloc_n(&d_tab, nch + nr, "pre-weights")) ||
+        (rc = dev_alloc_n(&d_tab, nch + nr + 2 * nsm, "pre-weights, source table")) ||
         (rc = dev_alloc_n(&d_cols, std::max<size_t>(1, cols.size()), "FOV columns")) ||
         (rc = dev_alloc_n(&d_off, g->nz, "slice offsets")) ||
         (rc = dev_alloc_n(&d_godd, nch, "ramp kernel"))) {
@@ -238,6 +304,8 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
     std::vector<float> goddf(godd.begin(), godd.end());
     if (e == hipSuccess) e = hipMemcpy(d_godd, goddf.data(), nch * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_tab + nch, pre_row.data(), nr * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d_tab + nch + nr, src.data(), 2 * nsm * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess && !cols.empty())
         e = hipMemcpy(d_cols, cols.data(), cols.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_off, off.data(), g->nz * 8, hipMemcpyHostToDevice);
@@ -250,21 +318,23 @@ extern "C" int ctbb_fbp(const ctbb_ssm *m, const float *raw, const int64_t *slic
     }
     if (e == hipSuccess && !cols.empty()) {
         const int64_t ncols = (int64_t)cols.size();
+        const uint32_t nvr = (uint32_t)(nch * nr), nturn32 = (uint32_t)nturn;
+        const float inv_nvr = (float)(1.0 / (double)nvr), inv_nturn = (float)(1.0 / (double)nturn);
         const unsigned grid = (unsigned)std::min<int64_t>((ncols + 3) / 4, 65536);
         if (m->val16)
             hipLaunchKernelGGL(fbp_backproject_kernel<true>, dim3(grid), dim3(kFbpBlock), 0, 0,
-                               m->col_ptr, m->idx, m->val, m->val16, m->col_scale,
-                               m->view_off, d_cols, ncols, g->nx, nxy, g->nz, nsm,
-                               g->center_voxel_x, g->center_voxel_y, g->voxel_size_x,
-                               g->voxel_size_y, Rs, g->tube_angle_increment, d_off,
-                               ds, nch * nr, nturn, d_q, d_x);
+                               m->col_ptr, m->idx, m->val, m->val16, m->col_scale, d_cols,
+                               ncols, g->nx, nxy, g->nz, g->center_voxel_x,
+                               g->center_voxel_y, g->voxel_size_x, g->voxel_size_y,
+                               d_tab + nch + nr, nsm, d_off, ds, nvr, inv_nvr, nturn32,
+                               inv_nturn, d_q, d_x);
         else
             hipLaunchKernelGGL(fbp_backproject_kernel<false>, dim3(grid), dim3(kFbpBlock), 0, 0,
-                               m->col_ptr, m->idx, m->val, m->val16, m->col_scale,
-                               m->view_off, d_cols, ncols, g->nx, nxy, g->nz, nsm,
-                               g->center_voxel_x, g->center_voxel_y, g->voxel_size_x,
-                               g->voxel_size_y, Rs, g->tube_angle_increment, d_off,
-                               ds, nch * nr, nturn, d_q, d_x);
+                               m->col_ptr, m->idx, m->val, m->val16, m->col_scale, d_cols,
+                               ncols, g->nx, nxy, g->nz, g->center_voxel_x,
+                               g->center_voxel_y, g->voxel_size_x, g->voxel_size_y,
+                               d_tab + nch + nr, nsm, d_off, ds, nvr, inv_nvr, nturn32,
+                               inv_nturn, d_q, d_x);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
