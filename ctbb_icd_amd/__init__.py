@@ -548,5 +548,17 @@ def copy_probe(nbytes: int) -> float:
     return ms.value
 
 
+_device = 0
+
+
 def init(device: int = 0) -> None:
+    """Select the HIP device for the calling host thread (HIP's current
+    device is per thread: worker threads call init(current_device()))."""
+    global _device
     check(lib().ctbb_init(device))
+    _device = device
+
+
+def current_device() -> int:
+    """The device the last init() selected."""
+    return _device

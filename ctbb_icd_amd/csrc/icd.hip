@@ -1641,10 +1641,15 @@ int red_grid(int64_t n)
                                                         4 * 1024));
 }
 
-// Build (once) the ray-major copy of the matrix.
+// Build (once) the ray-major copy of the matrix.  Reconstructions created
+// from several host threads share the matrix: the first builds the copy
+// while the others wait.
+static std::mutex g_ray_major_mu;
+
 int ensure_ray_major(const ctbb_ssm *mc)
 {
     ctbb_ssm *m = const_cast<ctbb_ssm *>(mc);
+    std::lock_guard<std::mutex> lock(g_ray_major_mu);
     if (m->rm_vc)
         return CTBB_OK;
     const ctbb_geom *g = &m->g;

@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Recon, SSM, Scan, UNet, rotate_slices
+from . import Recon, SSM, Scan, UNet, current_device, init, rotate_slices
 
 MU_WATER = 0.193   # 1/cm; inputs and outputs are scaled by 1/MU_WATER
 MU_MAX = 1.0       # 1/cm; network inputs and training images are clipped to [0, MU_MAX]
@@ -65,14 +65,17 @@ class Pair:
 
 
 def generate_pairs(make_scan, seeds, ssm: SSM, n_early: int, n_target: int,
-                   log=print, target_start: str = "zero") -> list[Pair]:
+                   log=print, target_start: str = "zero", workers: int = 1) -> list[Pair]:
     """make_scan(seed) -> Scan of the shared geometry with that phantom.
     Input: the image after n_early sweeps from zero.  Target: the image after
     n_target sweeps from zero (target_start "zero", the early sweeps
     included) or from the GPU FBP image ("fbp": closer to convergence for
-    the same number of sweeps)."""
-    pairs = []
-    for s in seeds:
+    the same number of sweeps).  workers > 1: that many phantoms at a time,
+    each from its own host thread on its own reconstruction (and so its own
+    HIP stream), which keeps the GPU busy where one small scan's launches
+    cannot; every run is deterministic, so the pairs are those of one
+    phantom at a time, returned in seed order."""
+    def one(s):
         t = time.perf_counter()
         scan = make_scan(s)
         raw, w = scan.synthesize(use_gpu=True)
@@ -86,9 +89,16 @@ def generate_pairs(make_scan, seeds, ssm: SSM, n_early: int, n_target: int,
             r.iterate(n_target - n_early)
         target = to_fixed(scan, r.volume())
         r.free()
-        pairs.append(Pair(s, early, target))
         log(f"[warmstart] pair seed {s}: {time.perf_counter() - t:.1f} s")
-    return pairs
+        return Pair(s, early, target)
+
+    seeds = list(seeds)
+    if workers <= 1:
+        return [one(s) for s in seeds]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=workers, initializer=init,
+                            initargs=(current_device(),)) as ex:
+        return list(ex.map(one, seeds))
 
 
 def save_params(path, net: UNet, meta: dict) -> None:
@@ -285,7 +295,7 @@ def _main():
     import argparse
     import json
     from pathlib import Path
-    from . import configs, cost as model_cost, init, set_verbose
+    from . import configs, cost as model_cost, set_verbose
 
     ap = argparse.ArgumentParser(description="train the warm-start network and "
                                  "measure time to cost tolerance, plain vs warm")
@@ -322,6 +332,8 @@ def _main():
     ap.add_argument("--target-start", choices=["zero", "fbp"], default="zero",
                     help="training targets: sweeps from zero or from the FBP image")
     ap.add_argument("--train-seed0", type=int, default=1000, help="first training phantom")
+    ap.add_argument("--workers", type=int, default=1,
+                    help="training pairs generated this many phantoms at a time")
     ap.add_argument("--save", default="", help="trained network as shipped (.npz)")
     ap.add_argument("--net", default="", help="network to evaluate (.npz from --save)")
     ap.add_argument("--converged", action="store_true",
@@ -380,9 +392,13 @@ def _main():
         ssm = SSM.build(g)
         log(f"[warmstart] train on {tcfg}: {g.nx}x{g.ny}x{g.nz}, SSM in "
             f"{time.perf_counter() - t:.1f} s")
+        t = time.perf_counter()
         pairs = generate_pairs(lambda sd: make_scan(tcfg, sd),
                                range(a.train_seed0, a.train_seed0 + a.train),
-                               ssm, a.early, a.target_its, log, a.target_start)
+                               ssm, a.early, a.target_its, log, a.target_start,
+                               a.workers)
+        t_pairs = time.perf_counter() - t
+        log(f"[warmstart] {len(pairs)} pairs in {t_pairs:.1f} s ({a.workers} at a time)")
         ssm.free()
         net = UNet(4, 32, a.batch, g.ny, g.nx, seed=11)
         t = time.perf_counter()
@@ -392,7 +408,7 @@ def _main():
         log(f"[warmstart] training {t_train:.1f} s; last step {ms:.2f} ms, "
             f"{flops / (ms * 1e-3) / 1e12:.1f} TFLOP/s in the 3x3 convolutions")
         summary.update(train=a.train, target_its=a.target_its, epochs=a.epochs,
-                       train_seconds=t_train, loss_history=hist, unet_step_ms=ms,
+                       pairs_seconds=t_pairs, workers=a.workers, train_seconds=t_train, loss_history=hist, unet_step_ms=ms,
                        unet_step_tflops=flops / (ms * 1e-3) / 1e12)
         if a.params_out:
             np.save(a.params_out, net.params())

@@ -155,3 +155,29 @@ def test_config3_pipeline_checks_the_start_and_reaches_target(ct, make_scan):
     bad.free()
     ident.free()
     ssm.free()
+
+
+def test_threaded_pairs_match_one_at_a_time(ct, tmp_path):
+    """generate_pairs with workers > 1 (several phantoms at once, one host
+    thread and one reconstruction each, sharing a matrix whose ray-major
+    copy the first reconstructions build concurrently) returns the pairs of
+    one phantom at a time, bit for bit and in seed order."""
+    from ctbb_icd_amd import configs, warmstart as ws
+    ct.init(0)
+
+    def make(seed):
+        return ct.Scan(configs.write_prm("tiny_ffs", tmp_path / f"p{seed}.yaml",
+                                         synthetic_seed=seed))
+
+    g = make(1).geom
+    seeds = [11, 12, 13, 14, 15]
+    ssm1 = ct.SSM.build(g)
+    serial = ws.generate_pairs(make, seeds, ssm1, 1, 3, lambda m: None, "fbp", workers=1)
+    ssm1.free()
+    ssm3 = ct.SSM.build(g)
+    threaded = ws.generate_pairs(make, seeds, ssm3, 1, 3, lambda m: None, "fbp", workers=3)
+    ssm3.free()
+    assert [p.seed for p in threaded] == seeds
+    for a, b in zip(serial, threaded):
+        assert np.array_equal(a.early, b.early) and np.array_equal(a.target, b.target)
+    assert not np.array_equal(serial[0].target, serial[1].target)
