@@ -577,7 +577,8 @@ __global__ __launch_bounds__(kSweepBlock) void icd_gather_kernel(SweepArgs a,
                     const uint32_t rc = in ? (uint32_t)r : 0u;
                     mv[m][u] = in ? av[u] : 0.0f;
                     ev[m][u] = ew[rc];
-                    wv[m][u] = (THETA2 && wt) ? wt[rc] : 1.0f;
+                    // the surrogate sums come weighted (w S); a^2 needs w
+                    wv[m][u] = (THETA2 && wt && !ssum) ? wt[rc] : 1.0f;
                     // curvature factor: the step's summed values on this
                     // sample (surrogate), or a itself (a^2 squared in FP32,
                     // as src/icd_iteration.cpp:208)
@@ -785,8 +786,10 @@ struct ScatterOrder {
     int64_t vlo, vhi, nq, ncg, P, rows;
 };
 
-// LDS: kScatterViews * kScatterCh * nr values of T (dynamic).
-template <typename T>
+// LDS: kScatterViews * kScatterCh * nr values of T (dynamic).  WS: the
+// first sweep's surrogate sums, written weighted, e = w (e - sum) with w =
+// a.w (the gather then reads w S as one array).
+template <typename T, bool WS = false>
 __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
     SweepArgs a, StepArgs sa, BtArgs bt, int64_t b, ScatterOrder so)
 {
@@ -915,7 +918,14 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
         const int64_t r0 = ((view0 + u) * bt.nch + c0) * bt.nr;
         const T *tacc = acc + u * tile;
         T *e = static_cast<T *>(a.e) + r0;
-        if (a.w) {
+        if (WS) {
+            const float *w = a.w + r0;
+            for (int i = tid; i < tile; i += kSweepBlock) {
+                const T d = tacc[i];
+                if (d != T(0))
+                    e[i] = (T)w[i] * (e[i] - d);
+            }
+        } else if (a.w) {
             T *ew = static_cast<T *>(a.ew) + r0;
             const float *w = a.w + r0;
             for (int i = tid; i < tile; i += kSweepBlock) {
@@ -3201,16 +3211,22 @@ static int recon_iterate_t(ctbb_recon *r, int n, ctbb_icd_stats *stats)
                     const int64_t nwg = so.P * so.rows * ncg;
                     scatter_grid = kXcd * ((nwg + kXcd - 1) / kXcd);
                 }
-                // first sweep with the batch surrogate: S += a over every voxel
-                // of the step (a scatter of -1 per voxel into the zeroed S)
+                // first sweep with the batch surrogate: S = sum of a over every
+                // voxel of the step (a scatter of -1 per voxel into the zeroed
+                // S), stored as w S when the data term is weighted
                 SweepArgs as = a;
                 as.e = r->ssum;
-                as.w = nullptr;
+                as.w = r->w;
                 if (first && r->ssum && scatter_grid > 0) {
                     as.du = r->du_m1;
-                    hipLaunchKernelGGL(icd_scatter_kernel<T>, dim3((unsigned)scatter_grid),
-                                       dim3(kSweepBlock), scatter_lds, r->stream, as, sa, bt,
-                                       b, so);
+                    if (r->w)
+                        hipLaunchKernelGGL((icd_scatter_kernel<T, true>),
+                                           dim3((unsigned)scatter_grid), dim3(kSweepBlock),
+                                           scatter_lds, r->stream, as, sa, bt, b, so);
+                    else
+                        hipLaunchKernelGGL((icd_scatter_kernel<T, false>),
+                                           dim3((unsigned)scatter_grid), dim3(kSweepBlock),
+                                           scatter_lds, r->stream, as, sa, bt, b, so);
                 }
                 const bool samp = (step % every) == 0 && used < kSamples;
                 if (samp)
