@@ -1631,20 +1631,6 @@ int upload_shear(const ctbb_geom *g, int32_t neighbours, const int64_t *slice_in
     return CTBB_OK;
 }
 
-int g_num_cu = 0;
-
-int num_cu()
-{
-    if (g_num_cu == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&g_num_cu, hipDeviceAttributeMultiprocessorCount,
-                                  dev) != hipSuccess)
-            g_num_cu = 64;
-    }
-    return g_num_cu;
-}
-
 int red_grid(int64_t n)
 {
     return (int)std::max<int64_t>(1, std::min<int64_t>((n + kRedBlock - 1) / kRedBlock,

@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) void to_fp16_kernel(
         red[threadIdx.x] = m;
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s)
+            if ((int)threadIdx.x < s)
                 red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
             __syncthreads();
         }

@@ -194,7 +194,6 @@ void load_system_matrix(const struct recon_params *rp, struct ct_data *data)
 
 void icd_iteration(const struct recon_params *rp, struct ct_data *data)
 {
-    ctbb_geom g = geom_of(rp);
     std::vector<int64_t> si = slice_index_of(rp, data);
     if (rp->wfbp_initialize && rp->initial_recon_path.compare("") == 0) {
         if (ctbb_fbp(data->ssm, data->raw, si.data(), data->recon_volume) != CTBB_OK)
