@@ -147,6 +147,38 @@ def cpu_baseline(scan, ssm, raw, w, target_s: float, threads: int):
     }
 
 
+def cpu_ssm_build(g, target_s: float):
+    """Time the reference's single-threaded ray-driven matrix build (the
+    oracle's restatement of src/generate_system_matrix_ffs.cpp:104-230) on a
+    sample of SSM views spread over the window; rays/s extrapolates to the
+    whole matrix (every view traces the same n_channels x n_rows rays, each
+    across the whole grid)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    n = g.n_views_sm
+    per_view = g.n_channels * g.n_rows
+    seen, elapsed, entries = set(), 0.0, 0
+    for i in range(n):
+        v = int(((i + 0.5) * 0.6180339887498949) % 1.0 * n)
+        if v in seen:
+            continue
+        seen.add(v)
+        _, idx, _, fill_s = orc.ssm_build_views(g, v, v + 1)
+        elapsed += fill_s
+        entries += len(idx)
+        if elapsed >= target_s:
+            break
+    rate = len(seen) * per_view / elapsed
+    return {
+        "value": rate, "unit": "rays/s", "cores": 1, "kind": "port",
+        "sample": f"{len(seen)} of {n} views (golden-ratio spread), "
+                  f"{len(seen) * per_view} rays, {entries} entries written in "
+                  f"{elapsed:.1f} s by the pass that traces and writes (the counting "
+                  f"pass that sizes the arrays untimed); whole matrix extrapolated "
+                  f"to {n * per_view / rate:.0f} s",
+    }
+
+
 def load_reference(config: str, seed: int):
     f = DATA / f"{config}_seed{seed}.npz"
     if not f.exists():
@@ -296,7 +328,7 @@ def main():
     rays = g.n_views_sm * g.n_channels * g.n_rows
     log(f"{args.config}: synth {t1 - t0:.1f}s, SSM nnz {nnz} ({nnz / (g.nx * g.ny):.0f}/column) "
         f"built in {build_ms:.0f} ms")
-    cpu = cpu_all = None
+    cpu = cpu_all = cpu_build = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(scan, ssm, raw, w, args.cpu_seconds, 11)
         log(f"cpu baseline (11 threads): {cpu['value']:.3e} updates/s ({cpu['sample']})")
@@ -304,6 +336,9 @@ def main():
         cpu_all = cpu_baseline(scan, ssm, raw, w, args.cpu_seconds, n_all)
         cpu_all["topology"] = cpu_topology()
         log(f"cpu baseline ({n_all} threads): {cpu_all['value']:.3e} updates/s")
+        cpu_build = cpu_ssm_build(g, 0.5 * args.cpu_seconds)
+        log(f"cpu matrix build (1 thread): {cpu_build['value']:.3e} rays/s "
+            f"({cpu_build['sample']})")
     m_run = ssm
     if fp16:
         m_run = ct.SSM.build(g)
@@ -435,6 +470,9 @@ def main():
                                    if build_flop else None),
                 "fp64_flop_source": "profiles/ssm_build_%s_counters.txt" % args.config
                                     if build_flop else None,
+                "cpu_baseline": cpu_build,
+                "gpu_over_cpu": (rays / (build_ms * 1e-3) / cpu_build["value"]
+                                 if cpu_build else None),
             },
             "cpu_baseline_all_cores": cpu_all,
             "cost_after": cost,

@@ -19,6 +19,10 @@
  *                        the deflection constants themselves
  *                        (src/generate_system_matrix_ffs.cpp:66-67,242-283)
  *                        are restated, parity unpinned.
+ *   orc_ssm_*_views      the ray-driven build over a view range: view range
+ *                        by view range, exactly the full build's entries of
+ *                        those views (it times the single-threaded CPU build
+ *                        on a sample of views).
  *   orc_forward_project  dense brute-force A x (numpy) on tiny inputs.
  *   orc_quadratic        reference src/penalties.cpp compiled from source
  *   orc_edge_preserving  (oracle/_ref), bit-exact; closed forms.
@@ -228,12 +232,13 @@ static double orc_da(const ctbb_geom *g)
 /* The ray loop of src/generate_system_matrix.cpp:80-186 (and its FFS twin
  * src/generate_system_matrix_ffs.cpp:104-222): emits (column, q, value) in
  * ascending q. */
-static int orc_ssm_rays(const ctbb_geom *g, emit_fn emit, void *ctx)
+static int orc_ssm_rays(const ctbb_geom *g, int v0, int v1, emit_fn emit,
+                        void *ctx)
 {
     int ffs = (g->zffs != 0 || g->phiffs != 0);
     double da = orc_da(g);
     int nx = (int)g->nx, ny = (int)g->ny;
-    for (int i = 0; i < (int)g->n_views_sm; i++) {
+    for (int i = v0; i < v1; i++) {
         view_geom v;
         orc_view(g, i, ffs, da, &v);
         for (int j = 0; j < (int)g->n_channels; j++) {
@@ -366,7 +371,7 @@ int64_t orc_ssm_count(const ctbb_geom *g, int64_t *col_cnt)
 {
     int64_t ncol = g->nx * g->ny;
     memset(col_cnt, 0, sizeof(int64_t) * ncol);
-    if (orc_ssm_rays(g, emit_count, col_cnt) != 0)
+    if (orc_ssm_rays(g, 0, (int)g->n_views_sm, emit_count, col_cnt) != 0)
         return -1;
     int64_t tot = 0;
     for (int64_t c = 0; c < ncol; c++)
@@ -385,7 +390,39 @@ int orc_ssm_fill(const ctbb_geom *g, const int64_t *col_ptr, int32_t *idx,
     memcpy(f.cursor, col_ptr, sizeof(int64_t) * ncol);
     f.idx = idx;
     f.val = val;
-    int rc = orc_ssm_rays(g, emit_fill, &f);
+    int rc = orc_ssm_rays(g, 0, (int)g->n_views_sm, emit_fill, &f);
+    free(f.cursor);
+    return rc;
+}
+
+int64_t orc_ssm_count_views(const ctbb_geom *g, int64_t v0, int64_t v1,
+                            int64_t *col_cnt)
+{
+    int64_t ncol = g->nx * g->ny;
+    if (v0 < 0 || v1 > g->n_views_sm || v0 > v1)
+        return -1;
+    memset(col_cnt, 0, sizeof(int64_t) * ncol);
+    orc_ssm_rays(g, (int)v0, (int)v1, emit_count, col_cnt);
+    int64_t tot = 0;
+    for (int64_t c = 0; c < ncol; c++)
+        tot += col_cnt[c];
+    return tot;
+}
+
+int orc_ssm_fill_views(const ctbb_geom *g, int64_t v0, int64_t v1,
+                       const int64_t *col_ptr, int32_t *idx, float *val)
+{
+    int64_t ncol = g->nx * g->ny;
+    if (v0 < 0 || v1 > g->n_views_sm || v0 > v1)
+        return -1;
+    fill_ctx f;
+    f.cursor = (int64_t *)malloc(sizeof(int64_t) * ncol);
+    if (!f.cursor)
+        return -1;
+    memcpy(f.cursor, col_ptr, sizeof(int64_t) * ncol);
+    f.idx = idx;
+    f.val = val;
+    int rc = orc_ssm_rays(g, (int)v0, (int)v1, emit_fill, &f);
     free(f.cursor);
     return rc;
 }

@@ -25,6 +25,17 @@ int64_t orc_ssm_count(const ctbb_geom *g, int64_t *col_cnt);
 int orc_ssm_fill(const ctbb_geom *g, const int64_t *col_ptr, int32_t *idx,
                  float *val);
 
+/* The ray-driven build restricted to SSM views [v0, v1): the same rays,
+ * crossings and entries as orc_ssm_count/fill over those views (their q
+ * keep the full-matrix numbering), so the matrix is the concatenation, view
+ * range by view range, of these pieces per column.  Returns -1 for a range
+ * outside [0, n_views_sm).  Used to time the reference's single-threaded
+ * build on a sample of views. */
+int64_t orc_ssm_count_views(const ctbb_geom *g, int64_t v0, int64_t v1,
+                            int64_t *col_cnt);
+int orc_ssm_fill_views(const ctbb_geom *g, int64_t v0, int64_t v1,
+                       const int64_t *col_ptr, int32_t *idx, float *val);
+
 /* A3/A4 for a subset of columns, voxel-driven (the same per-crossing
  * expressions as the ray loop, every ray of every view screened): pass 1
  * writes cnt[t] = entries of column cols[t] and returns the total; pass 2

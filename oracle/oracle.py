@@ -44,6 +44,8 @@ def lib() -> C.CDLL:
         sig = {
             "orc_ssm_count": ([G, _P], _i64),
             "orc_ssm_fill": ([G, _P, _P, _P], C.c_int),
+            "orc_ssm_count_views": ([G, _i64, _i64, _P], _i64),
+            "orc_ssm_fill_views": ([G, _i64, _i64, _P, _P, _P], C.c_int),
             "orc_forward_project": ([G, _P, _P, _P, _P, _P, _P], None),
             "orc_quadratic": ([G, _i64, _f64, _f64, _f64, _P], _f64),
             "orc_edge_preserving": ([G, _i64, _f64, _f64, _f64, _f64, _P], _f64),
@@ -107,6 +109,26 @@ def ssm_build(geom):
     val = np.zeros(max(nnz, 1), np.float32)
     lib().orc_ssm_fill(gp(geom), _p(cp), _p(idx), _p(val))
     return cp, idx[:nnz], val[:nnz]
+
+
+def ssm_build_views(geom, v0, v1):
+    """The ray-driven build over SSM views [v0, v1) only: (ptr, idx, val,
+    fill_s), entries numbered as in the full matrix; fill_s is the wall time
+    of the pass that traces the rays and writes the entries (the counting
+    pass that sizes the arrays is not included)."""
+    import time
+    ncol = geom.nx * geom.ny
+    cnt = np.zeros(ncol, np.int64)
+    nnz = lib().orc_ssm_count_views(gp(geom), v0, v1, _p(cnt))
+    if nnz < 0:
+        raise ValueError(f"oracle SSM: view range [{v0}, {v1}) outside the matrix")
+    cp = np.zeros(ncol + 1, np.int64)
+    cp[1:] = np.cumsum(cnt)
+    idx = np.zeros(max(nnz, 1), np.int32)
+    val = np.zeros(max(nnz, 1), np.float32)
+    t0 = time.perf_counter()
+    lib().orc_ssm_fill_views(gp(geom), v0, v1, _p(cp), _p(idx), _p(val))
+    return cp, idx[:nnz], val[:nnz], time.perf_counter() - t0
 
 
 def ssm_cols(geom, cols):

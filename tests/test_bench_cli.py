@@ -14,3 +14,17 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+def test_cpu_matrix_build_leg_runs_on_a_view_sample(make_scan):
+    """The single-threaded matrix-build baseline times whole views of the
+    ray-driven build and reports rays/s over the views it traced."""
+    sys.path.insert(0, str(ROOT))
+    import bench
+    g = make_scan("tiny_ffs").geom
+    r = bench.cpu_ssm_build(g, 0.05)
+    assert r["unit"] == "rays/s" and r["cores"] == 1 and r["kind"] == "port"
+    assert r["value"] > 0
+    nv = int(r["sample"].split()[0])
+    assert 1 <= nv <= g.n_views_sm
+    assert f"{nv * g.n_channels * g.n_rows} rays" in r["sample"]

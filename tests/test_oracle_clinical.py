@@ -9,6 +9,10 @@
     channel each way: the value-weighted channel centroids of even and odd
     views differ by half a channel (the interlace the deflection exists
     for).  A dropped, halved or sign-flipped da fails it.
+  * The ray-driven build over a range of views (the CPU baseline of the
+    matrix build times it on a view sample) gives, range by range, exactly
+    the full build's entries of those views, and rejects ranges outside the
+    matrix.
   * The footprint-overlap helper equals a brute-force set intersection.
   * The OpenMP CPU baseline (the reference's loop structure) computes what the
     sequential oracle computes in the reference order.
@@ -48,6 +52,29 @@ def test_column_subset_ssm_equals_ray_driven_build(make_scan, orc, name):
         assert ptr[t + 1] - ptr[t] == b - a, (name, c)
         assert np.array_equal(cidx[ptr[t]:ptr[t + 1]], idx[a:b]), (name, c)
         assert np.array_equal(cval[ptr[t]:ptr[t + 1]], val[a:b]), (name, c)
+
+
+@pytest.mark.parametrize("name", ["tiny", "tiny_ffs", "tiny_zphiffs"])
+def test_view_range_build_is_a_piece_of_the_full_build(make_scan, orc, name):
+    g = make_scan(name).geom
+    cp, idx, val = orc.ssm_build(g)
+    n = g.n_views_sm
+    cuts = [0, 1, n // 3, n // 3 + 2, n - 1, n]
+    pieces = [orc.ssm_build_views(g, a, b) for a, b in zip(cuts[:-1], cuts[1:])]
+    per_view = g.n_rows * g.n_channels
+    for c in range(g.nx * g.ny):
+        ref_i, ref_v = idx[cp[c]:cp[c + 1]], val[cp[c]:cp[c + 1]]
+        got_i = np.concatenate([p[1][p[0][c]:p[0][c + 1]] for p in pieces])
+        got_v = np.concatenate([p[2][p[0][c]:p[0][c + 1]] for p in pieces])
+        assert np.array_equal(got_i, ref_i), (name, c)
+        assert np.array_equal(got_v, ref_v), (name, c)
+    for (a, b), p in zip(zip(cuts[:-1], cuts[1:]), pieces):
+        q = p[1].astype(np.int64)
+        assert np.all((q >= a * per_view) & (q < b * per_view)), (name, a, b)
+    with pytest.raises(ValueError):
+        orc.ssm_build_views(g, n - 1, n + 1)
+    with pytest.raises(ValueError):
+        orc.ssm_build_views(g, 2, 1)
 
 
 @pytest.mark.parametrize("name,cols", [
