@@ -6,7 +6,9 @@
 // State (device resident, reference sinogram layout
 // r = row + nr*(channel + nch*view)):
 //   e   FP32 error sinogram y - A x (the reference keeps yhat = A x in FP64
-//       and forms raw - yhat on the fly, src/icd_iteration.cpp:209)
+//       and forms raw - yhat on the fly, src/icd_iteration.cpp:209); in
+//       weighted sweeps kept current only where w == 0 and formed as w e / w
+//       elsewhere when it is read (running cost, download)
 //   ew  FP32 weighted error w e (weighted reconstructions; e itself when
 //       unweighted), the only sinogram the gather reads
 //   x   FP32 volume in the rotated frame, index c + nx*ny*k
@@ -926,13 +928,18 @@ __global__ __launch_bounds__(kSweepBlock) void icd_scatter_kernel(
                     e[i] = (T)w[i] * (e[i] - d);
             }
         } else if (a.w) {
+            // only w e is kept current (the gather reads nothing else); e
+            // follows where w == 0, where w e cannot give it back, and is
+            // formed as w e / w elsewhere when it is read (refresh_error)
             T *ew = static_cast<T *>(a.ew) + r0;
             const float *w = a.w + r0;
             for (int i = tid; i < tile; i += kSweepBlock) {
                 const T d = tacc[i];
                 if (d != T(0)) {
-                    e[i] = e[i] - d;
-                    ew[i] = ew[i] - (T)w[i] * d;
+                    const float wi = w[i];
+                    ew[i] = ew[i] - (T)wi * d;
+                    if (wi == 0.0f)
+                        e[i] = e[i] - d;
                 }
             }
         } else {
@@ -1215,6 +1222,20 @@ __global__ __launch_bounds__(kRedBlock) void weighted_error_kernel(
     for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kRedBlock)
         ew[i] = (T)((double)w[i] * (double)e[i]);
+}
+
+// e = w e / w wherever w > 0 (the weighted sweep keeps only w e current
+// there); samples with w == 0 keep their e.
+template <typename T>
+__global__ __launch_bounds__(kRedBlock) void error_from_weighted_kernel(
+    T *e, const T *ew, const float *w, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kRedBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kRedBlock) {
+        const float wi = w[i];
+        if (wi > 0.0f)
+            e[i] = (T)((double)ew[i] / (double)wi);
+    }
 }
 
 // out = (T) in: the measured sinogram as the starting error sinogram.
@@ -2800,10 +2821,30 @@ extern "C" int ctbb_recon_get_volume64(const ctbb_recon *r, double *volume)
     return state_download(r, volume, r->x, r->nvox, 8);
 }
 
+// Bring the error sinogram e up to date from w e (weighted sweeps update
+// e only where w == 0).
+static int refresh_error(const ctbb_recon *r)
+{
+    if (!r->w)
+        return CTBB_OK;
+    const int g = red_grid(r->data_size);
+    if (r->fp64)
+        hipLaunchKernelGGL(error_from_weighted_kernel<double>, dim3(g), dim3(kRedBlock), 0,
+                           r->stream, static_cast<double *>(r->e),
+                           static_cast<const double *>(r->ew), r->w, r->data_size);
+    else
+        hipLaunchKernelGGL(error_from_weighted_kernel<float>, dim3(g), dim3(kRedBlock), 0,
+                           r->stream, static_cast<float *>(r->e),
+                           static_cast<const float *>(r->ew), r->w, r->data_size);
+    CTBB_HIP(hipGetLastError());
+    return CTBB_OK;
+}
+
 extern "C" int ctbb_recon_get_error(const ctbb_recon *r, float *e)
 {
     if (!r || !e)
         return fail(CTBB_E_ARG, "null argument");
+    CTBB_TRY(refresh_error(r));
     return state_download(r, e, r->e, r->data_size, 4);
 }
 
@@ -3054,6 +3095,7 @@ extern "C" int ctbb_recon_cost_running(ctbb_recon *r, double *cost)
         return fail(CTBB_E_ARG, "null argument");
     double data = 0.0, pen = 0.0;
     const int g = red_grid(r->data_size);
+    CTBB_TRY(refresh_error(r));
     if (r->fp64)
         hipLaunchKernelGGL(error_sq_kernel<double>, dim3(g), dim3(kRedBlock), 0, r->stream,
                            static_cast<const double *>(r->e), r->w, r->data_size,
