@@ -183,6 +183,8 @@ struct ctbb_recon {
     bool fp64 = false;
     size_t sb = 4;
     void *e = nullptr;          // running error sinogram y - A x
+    mutable bool e_stale = false;  // weighted sweeps ran since e was formed:
+                                   // e is current only where w == 0
     void *x = nullptr;          // volume, rotated frame
     int64_t *slice_off = nullptr;  // [nz] sample offsets
     double *shear = nullptr;    // [2*(nz-1)] 26-neighbour slice rotations (or null)

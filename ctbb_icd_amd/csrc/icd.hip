@@ -1876,6 +1876,8 @@ int recon_data_term_t(ctbb_recon *r, bool write_e, double *data)
                                       r->k_lo, r->k_hi, nullptr, r->y, r->w,
                                       write_e ? e : nullptr, write_e ? ew : nullptr,
                                       r->partials, r->stream)));
+    if (write_e)
+        r->e_stale = false;
     return sum_partials(r->partials, g, r->stream, data);
 }
 
@@ -2825,7 +2827,7 @@ extern "C" int ctbb_recon_get_volume64(const ctbb_recon *r, double *volume)
 // e only where w == 0).
 static int refresh_error(const ctbb_recon *r)
 {
-    if (!r->w)
+    if (!r->w || !r->e_stale)
         return CTBB_OK;
     const int g = red_grid(r->data_size);
     if (r->fp64)
@@ -2837,6 +2839,7 @@ static int refresh_error(const ctbb_recon *r)
                            r->stream, static_cast<float *>(r->e),
                            static_cast<const float *>(r->ew), r->w, r->data_size);
     CTBB_HIP(hipGetLastError());
+    r->e_stale = false;
     return CTBB_OK;
 }
 
@@ -3288,6 +3291,8 @@ static int recon_iterate_t(ctbb_recon *r, int n, ctbb_icd_stats *stats)
             }
         }
         r->th2_valid = true;
+        if (r->w)
+            r->e_stale = true;
         CTBB_HIP(hipGetLastError());
         CTBB_HIP(hipEventRecord(e1, r->stream));
         CTBB_HIP(hipEventSynchronize(e1));
