@@ -215,6 +215,33 @@ def test_running_error_tracks_fresh_projection(gpu, ssm_pair, orc):
     assert np.abs(e_run - (raw - y)).max() <= 1e-4 * np.abs(raw).max()
 
 
+def test_weighted_running_error_tracks_fresh_projection(gpu, ssm_pair, orc):
+    """Weighted sweeps keep w e current everywhere and e itself only where
+    w == 0; the downloaded e (formed as w e / w where w > 0) equals y - A x of
+    the FP64 projection on both kinds of samples."""
+    s, dev, host = ssm_pair("tiny_ffs")
+    raw, w = s.synthesize(use_gpu=False)
+    w = w.copy()
+    w[::7] = 0.0
+    zero = w == 0.0
+    r = gpu.Recon(dev, raw, w, s.slice_index, s.opts(tile=4))
+    r.iterate(3)
+    e_run = r.error().astype(np.float64)
+    y = orc.forward_project(s.geom, host, s.slice_index, r.volume().astype(np.float64))
+    ref = raw.astype(np.float64) - y
+    scale = np.abs(raw).max()
+    err_w, err_0 = (np.abs(e_run - ref)[~zero].max(), np.abs(e_run - ref)[zero].max())
+    print(f"[weighted-error] max |e - (y - Ax)|: w > 0 {err_w:.2e}, w == 0 {err_0:.2e} "
+          f"({zero.sum()} samples)", flush=True)
+    # e started as y (x = 0): the sweeps moved it on the w == 0 samples too
+    assert np.abs(e_run[zero] - raw[zero]).max() > 0.1 * scale
+    assert err_w <= 1e-4 * scale and err_0 <= 1e-4 * scale
+    # the running cost is formed from the same e
+    c64, c32 = r.cost(), r.cost_running()
+    assert abs(c32 - c64) <= 1e-5 * abs(c64)
+    r.free()
+
+
 def test_synthetic_generator_gpu_matches_cpu(gpu, make_scan):
     s = make_scan("tiny")
     a, _ = s.synthesize(use_gpu=False)
