@@ -16,7 +16,9 @@ timed region.  Prints one JSON line.
                      + 4 B weight read [scatter], plus each FOV column's
                      matrix entries once per sweep) over its average launch
                      time, measured with HIP events on the reconstruction's
-                     stream; traffic from the committed rocprofv3 counters
+                     stream; traffic (HBM bytes per launch) from the
+                     committed rocprofv3 counters, and the HBM rate it gives
+                     at the measured launch time
   cpu_baseline       the reference's OpenMP ICD structure (the oracle's
                      restatement) on this host, 11 threads (the reference's
                      team), on a band of rows of the same workload
@@ -413,6 +415,10 @@ def main():
         r.free()
     else:
         cost = (ttt or {}).get("plain_fp32", {}).get("cost")
+    # measured HBM bytes per launch of the dominant kernel (committed counter
+    # summary of this workload at the default schedule and FP32 matrix)
+    traffic = (measured_traffic(args.config, dom)
+               if not fp16 and args.tile == 0 else None)
     build_bytes = nnz * 8 + g.nx * g.ny * g.n_views_sm * 4
     build_flop = ssm_build_flop(args.config)
     out = {
@@ -443,8 +449,9 @@ def main():
             "bound": "hbm", "kernel": dom,
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (measured_traffic(args.config, dom)
-                        if not fp16 and args.tile == 0 else None),
+            "traffic": traffic,
+            "traffic_GB_per_s": (traffic / (dom_ms * 1e-3) / 1e9
+                                 if traffic and dom_ms else None),
             "algorithmic_bytes_per_launch": dom_bytes,
             "avg_launch_ms": dom_ms,
         },
