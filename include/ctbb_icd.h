@@ -278,7 +278,11 @@ int ctbb_recon_cost(ctbb_recon *r, double *cost);
  * rounded FP64 projection residual); for cheap comparisons of candidate
  * volumes and for monitoring.  No reference counterpart. */
 int ctbb_recon_cost_running(ctbb_recon *r, double *cost);
-/* Copy the running error sinogram to the host (rounded to FP32). */
+/* Copy the running error sinogram to the host (rounded to FP32).  Weighted
+ * sweeps keep w e current and e itself only where w == 0; after them, this
+ * call and ctbb_recon_cost_running first form e = w e / w where w > 0 (one
+ * pass over the sinogram on the reconstruction's stream).  No reference
+ * counterpart (the reference keeps yhat = A x, src/icd_iteration.cpp:47-48). */
 int ctbb_recon_get_error(const ctbb_recon *r, float *e);
 
 /* FP64 cost of a volume (rotated frame, float32 [nx*ny*nz]) for raw data,
