@@ -2,7 +2,8 @@
 built, to the reference's own penalty/rotation code (oracle/_ref).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.
+bench.py's CPU baseline legs (the ICD and the matrix build), as the checker
+or the timed CPU baseline.
 """
 from __future__ import annotations
 
