@@ -3,8 +3,9 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the ctbb_icd_amd
  * library, its C ABI or its driver) links, loads or calls this code; only
- * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
- * as the checker.  Every function cites the reference lines it restates.
+ * tests/, __graft_entry__.smoke() and bench.py's CPU baseline legs (ICD and
+ * matrix build) use it, as the checker or the timed CPU baseline.  Every
+ * function cites the reference lines it restates.
  *
  * Parity status of each function (what pins it) is listed in oracle.c.
  */
