@@ -9,6 +9,9 @@ the reference's key set.
   tiny, tiny_ffs   test geometries the CPU oracle runs in well under a second
   tiny_zffs,       tiny_ffs with the z flying focal spot (16 detector rows,
   tiny_zphiffs     8 per projection), and with both spots
+  tiny_cone        tiny with 10 cm detector rows (cone half-angle up to
+                   17.9 degrees at pitch ~1) and a uniform water cylinder:
+                   the FBP's cone weight changes its value by 2 %
   config0          16 x 368 detector, 576 views/turn, 128x128x32 volume,
                    4 rotations (2 fail the reference's own z-range check),
                    quadratic lambda 1e3, weights = counts, I0 = 1e5
@@ -116,6 +119,10 @@ CONFIGS["config4"] = dict(CONFIGS["config1"], Readings=13824, z_start=-16.5,
                           synthetic_weights=1)
 CONFIGS["config4_slab"] = dict(CONFIGS["config4"], Readings=4608, z_start=-4.0,
                                z_end=-2.05)
+CONFIGS["tiny_cone"] = dict(
+    CONFIGS["tiny"], axial_detector_spacing=10.0, table_feed_per_rotation=43.0,
+    z_start=-40.0, z_end=-30.0, slice_thickness=1.0, CollSlicewidth=1.0,
+    synthetic_n_ellipsoids=0)
 CONFIGS["clin_crop"] = dict(
     CONFIGS["config1"], Readings=1152, n_channels=92,
     num_views_per_turn_without_ffs=192, center_channel_non_ffs=45.25,

@@ -24,8 +24,9 @@ cone weight, backprojected through the stored system matrix:
 
 Pins (tests/test_fbp_oracle.py): water cylinders return mu inside and zero
 outside the FOV; on the config 0 scanner a 10 cm cylinder resolves the ramp
-constant, cos(gamma), 1/L^2 and the redundancy weights.  The FDK cone weight
-is parity unpinned: it moves those checks by < 0.01 %.
+constant, cos(gamma), 1/L^2 and the redundancy weights; on a scan with 10 cm
+detector rows (cone half-angle up to 17.9 degrees) the z-invariant cylinder
+resolves the FDK cone weight (without it the interior reads 2.1 % high).
 """
 from __future__ import annotations
 

@@ -345,3 +345,23 @@ def test_fbp_matches_oracle(gpu, make_scan, orc, name):
     print(f"[fbp] {name}: cost zero start {c0:.6e}, FBP start {c1:.6e}", flush=True)
     assert c1 < 0.8 * c0
     r.free()
+
+
+def test_fbp_wide_cone_matches_oracle(gpu, ssm_pair, orc):
+    """The wide-cone scan that pins the FBP's cone weight (10 cm detector
+    rows, cone half-angle up to 17.9 degrees): the device matrix equals the
+    oracle's and the device FBP equals the restatement."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent / "oracle"))
+    import fbp_oracle
+    s, dev, (cp, idx, val) = ssm_pair("tiny_cone")
+    gcp, gidx, gval = dev.download()
+    assert np.array_equal(gcp, cp) and np.array_equal(gidx, idx)
+    assert np.array_equal(gval, val)
+    raw, _ = s.synthesize(use_gpu=False)
+    xg = dev.fbp(raw, s.slice_index).astype(np.float64)
+    xo = fbp_oracle.fbp(s.geom, (cp, idx, val), raw, s.slice_index)
+    rms = np.sqrt(np.mean((xg - xo) ** 2)) / np.sqrt(np.mean(xo ** 2))
+    print(f"[fbp] tiny_cone: rel RMS vs oracle {rms:.2e}", flush=True)
+    assert rms <= 1e-4
