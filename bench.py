@@ -116,8 +116,11 @@ def cpu_topology() -> str:
                      for line in out.splitlines() if line.split(":", 1)[0].strip() in keep)
 
 
-def cpu_baseline(scan, ssm, raw, w, target_s: float, threads: int):
-    """Time the reference-structured OpenMP ICD (oracle) on a band of rows."""
+def cpu_baseline(scan, ssm, raw, w, target_s: float, threads: int, streamed: bool = False):
+    """Time the reference-structured OpenMP ICD (oracle) on a band of rows,
+    the matrix held in RAM; streamed: also on the same band with the matrix
+    read from a matrix.bin file of those rows every iteration, as the
+    reference reads it (src/icd_iteration.cpp:164-179)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as orc
     g = scan.geom
@@ -140,13 +143,28 @@ def cpu_baseline(scan, ssm, raw, w, target_s: float, threads: int):
         if elapsed >= 0.3 * target_s or y0 + 2 * rows > ny:
             break
         rows = min(ny - y0, max(rows + 1, int(rows * target_s / max(elapsed, 1e-3))))
-    return {
+    out = {
         "value": upd / elapsed, "unit": "voxel updates/s", "cores": threads,
         "kind": "port",
         "sample": f"{rows} row(s) of voxel columns (iy {y0}..{y0 + rows - 1}, all x, "
                   f"all {g.nz} slices), first iteration from zero, "
                   f"{upd} updates in {elapsed:.1f} s; matrix held in RAM",
     }
+    if streamed:
+        with tempfile.TemporaryDirectory() as d:
+            f = Path(d) / "band_matrix.bin"
+            orc.write_band_file(f, g, (cp, idx, val), y0, y0 + rows)
+            t = time.perf_counter()
+            _, _, upd_f = orc.icd_ref_omp_file(g, f, raw, w, scan.slice_index,
+                                               scan.run.penalty, scan.run.lambda_,
+                                               scan.run.delta, 1, threads, y0, y0 + rows)
+            el_f = time.perf_counter() - t
+            out["streamed"] = {
+                "value": upd_f / el_f, "unit": "voxel updates/s",
+                "sample": f"the same rows, matrix read from a matrix.bin file of them "
+                          f"({f.stat().st_size / 2**20:.0f} MiB, just written: page "
+                          f"cache) during the sweep, {upd_f} updates in {el_f:.1f} s"}
+    return out
 
 
 def cpu_ssm_build(g, target_s: float):
@@ -332,8 +350,10 @@ def main():
         f"built in {build_ms:.0f} ms")
     cpu = cpu_all = cpu_build = None
     if args.cpu_seconds > 0:
-        cpu = cpu_baseline(scan, ssm, raw, w, args.cpu_seconds, 11)
+        cpu = cpu_baseline(scan, ssm, raw, w, args.cpu_seconds, 11, streamed=True)
         log(f"cpu baseline (11 threads): {cpu['value']:.3e} updates/s ({cpu['sample']})")
+        log(f"cpu baseline (11 threads, matrix streamed from a file): "
+            f"{cpu['streamed']['value']:.3e} updates/s ({cpu['streamed']['sample']})")
         n_all = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         cpu_all = cpu_baseline(scan, ssm, raw, w, args.cpu_seconds, n_all)
         cpu_all["topology"] = cpu_topology()

@@ -135,6 +135,19 @@ int64_t orc_icd_ref_omp(const ctbb_geom *g, const int64_t *col_ptr,
                         int threads, int64_t row_begin, int64_t row_end,
                         double *x, double *yhat);
 
+/* The same sweep with the matrix read from a matrix.bin-format file on every
+ * iteration, as src/icd_iteration.cpp:164-179,270-273 reads it: the file
+ * holds the columns of rows [row_begin, row_end) (x fastest), each a uint64
+ * nnz and nnz {int32 index, float32 value} pairs; every column is read into
+ * a fresh array, inside the FOV or not, and the file is rewound after each
+ * iteration.  Returns the update count, or -1 if the file cannot be opened
+ * or ends early. */
+int64_t orc_icd_ref_omp_file(const ctbb_geom *g, const char *path, const float *raw,
+                             const float *weights, const int64_t *slice_index,
+                             int penalty, double lambda, double delta, int n_iter,
+                             int threads, int64_t row_begin, int64_t row_end,
+                             double *x, double *yhat);
+
 /* A9: global cost of x (computes a fresh A x in FP64). */
 double orc_cost(const ctbb_geom *g, const int64_t *col_ptr,
                 const int32_t *idx, const float *val, const float *raw,

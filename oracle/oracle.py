@@ -59,6 +59,8 @@ def lib() -> C.CDLL:
             "orc_column_overlaps": ([G, _P, _P, _i64, _i64], C.c_int),
             "orc_icd_ref_omp": ([G, _P, _P, _P, _P, _P, _P, C.c_int, _f64, _f64,
                                  C.c_int, C.c_int, _i64, _i64, _P, _P], _i64),
+            "orc_icd_ref_omp_file": ([G, C.c_char_p, _P, _P, _P, C.c_int, _f64, _f64,
+                                      C.c_int, C.c_int, _i64, _i64, _P, _P], _i64),
             "orc_icd_trace": ([G, _P, _P, _P, _P, _P, _P, C.c_int, _f64, _f64, _i64,
                                _i64, C.c_int, C.c_int, _P, _P, _P, _i64, _P, _P], _i64),
             "orc_ssm_cols_count": ([G, _i64, _P, _P], _i64),
@@ -212,6 +214,40 @@ def icd_ref_omp(geom, ssm, raw, weights, slice_index, penalty, lam, delta,
                               _p(np.ascontiguousarray(slice_index, np.int64)),
                               penalty, lam, delta, n_iter, threads, row_begin,
                               row_end, _p(x), _p(yhat))
+    return x, yhat, n
+
+
+def write_band_file(path, geom, ssm, row_begin, row_end):
+    """The columns of rows [row_begin, row_end) (x fastest) of a CSC matrix
+    (col_ptr, idx, val) in the matrix.bin layout: per column a uint64 nnz and
+    nnz {int32 index, float32 value} pairs.  col_ptr may cover the whole
+    matrix or be zero outside the band."""
+    cp, idx, val = ssm
+    pair = np.dtype([("index", "<i4"), ("value", "<f4")])
+    with open(path, "wb") as f:
+        for c in range(row_begin * geom.nx, row_end * geom.nx):
+            a, b = int(cp[c]), int(cp[c + 1])
+            f.write(np.uint64(b - a).tobytes())
+            rec = np.empty(b - a, pair)
+            rec["index"], rec["value"] = idx[a:b], val[a:b]
+            f.write(rec.tobytes())
+
+
+def icd_ref_omp_file(geom, path, raw, weights, slice_index, penalty, lam, delta,
+                     n_iter, threads, row_begin, row_end, x=None, yhat=None):
+    """orc_icd_ref_omp with the matrix read from `path` (write_band_file of
+    the same rows) on every iteration."""
+    x = np.zeros(geom.nvox, np.float64) if x is None else np.array(x, np.float64)
+    yhat = np.zeros(geom.data_size, np.float64) if yhat is None else np.array(yhat)
+    raw = np.ascontiguousarray(raw, np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    n = lib().orc_icd_ref_omp_file(gp(geom), str(path).encode(), _p(raw), _p(w),
+                                   _p(np.ascontiguousarray(slice_index, np.int64)),
+                                   penalty, lam, delta, n_iter, threads, row_begin,
+                                   row_end, _p(x), _p(yhat))
+    if n < 0:
+        raise IOError(f"matrix file {path} missing or shorter than rows "
+                      f"[{row_begin}, {row_end})")
     return x, yhat, n
 
 

@@ -15,7 +15,8 @@
     matrix.
   * The footprint-overlap helper equals a brute-force set intersection.
   * The OpenMP CPU baseline (the reference's loop structure) computes what the
-    sequential oracle computes in the reference order.
+    sequential oracle computes in the reference order, with the matrix held in
+    RAM or read from a matrix.bin file every iteration.
   * The batch surrogate on the clinical cone geometry with the library's
     default slice spacing (footprint conflicts inside <= 1 % of the columns):
     every step of a sweep lowers the cost although the step's voxels share
@@ -137,6 +138,35 @@ def test_openmp_baseline_equals_sequential_oracle(make_scan, orc, name):
     # the OpenMP reductions add in another order: equal to ~1e-12
     assert np.abs(xo - xs).max() <= 1e-12 * max(1.0, np.abs(xs).max())
     assert np.abs(yo - ys).max() <= 1e-12 * max(1.0, np.abs(ys).max())
+
+
+@pytest.mark.parametrize("name", ["tiny", "tiny_ffs"])
+def test_streamed_baseline_equals_in_ram_baseline(make_scan, orc, name, tmp_path):
+    """The CPU baseline with the matrix read from a matrix.bin file on every
+    iteration (as the reference reads it) computes what the in-RAM form
+    computes on the same band of rows; a file shorter than the band is an
+    error."""
+    s = make_scan(name)
+    g = s.geom
+    ssm = orc.ssm_build(g)
+    raw, w = s.synthesize(use_gpu=False)
+    pen, lam, delta = s.run.penalty, s.run.lambda_, s.run.delta
+    r0, r1 = g.ny // 4, g.ny // 4 + 6
+    f = tmp_path / "band.bin"
+    orc.write_band_file(f, g, ssm, r0, r1)
+    assert f.stat().st_size == 8 * (r1 - r0) * g.nx + 8 * int(ssm[0][r1 * g.nx]
+                                                              - ssm[0][r0 * g.nx])
+    xr, yr, nr = orc.icd_ref_omp(g, ssm, raw, w, s.slice_index, pen, lam, delta, 2, 3,
+                                 r0, r1)
+    xf, yf, nf = orc.icd_ref_omp_file(g, f, raw, w, s.slice_index, pen, lam, delta, 2, 3,
+                                      r0, r1)
+    assert nf == nr > 0
+    assert np.abs(xf - xr).max() <= 1e-12 * max(1.0, np.abs(xr).max())
+    assert np.abs(yf - yr).max() <= 1e-12 * max(1.0, np.abs(yr).max())
+    short = tmp_path / "short.bin"
+    short.write_bytes(f.read_bytes()[:-4])
+    with pytest.raises(IOError):
+        orc.icd_ref_omp_file(g, short, raw, w, s.slice_index, pen, lam, delta, 1, 3, r0, r1)
 
 
 def _default_spacing(orc, g, ssm, slice_index, frac=0.01):
