@@ -243,8 +243,9 @@ typedef struct ctbb_icd_stats {
 /* raw: measured line integrals y [readings*n_channels*n_rows], reference
  * layout raw[row + n_rows*channel + n_rows*n_channels*view]
  * (src/setup.cpp:335-344).  weights: per-sample statistical weights of the
- * same layout, or NULL (unweighted, as the reference).  slice_index[nz]:
- * central view of every slice (multiple of the FFS count). */
+ * same layout, finite and >= 0 (e.g. detected counts; not checked), or NULL
+ * (unweighted, as the reference).  slice_index[nz]: central view of every
+ * slice (multiple of the FFS count). */
 int ctbb_recon_create(const ctbb_ssm *m, const float *raw,
                       const float *weights, const int64_t *slice_index,
                       const ctbb_icd_opts *opts, ctbb_recon **out);
